@@ -1,0 +1,109 @@
+"""Closed-form initial data used by the parity tests and the benchmark.
+
+TEST INFRASTRUCTURE ONLY (see oracle/bssn.py).  The product library has its
+own copy of the analytic state builders in ``jax_bssn_amd/initial_data.py``;
+this module restates the reference formulas independently so that the two
+can be compared.
+
+* gauge_wave_analytic_state      tests/initial_data.py:147-205
+* periodic_coordinate_arrays     tests/initial_data.py:62-73
+* periodic_gauge_wave_state      tests/initial_data.py:76-144
+* brill_lindquist_conformal_factor  utilities/bowen_york_solver.py:52-63
+"""
+
+import numpy as np
+
+from .bssn import (
+    BSSNVariables,
+    christoffel_symbols_second_kind,
+    invert_3x3_metric,
+    trace_tensor,
+    traceless_part,
+)
+from .derivatives import diff1_field
+
+
+def periodic_coordinate_arrays(grid_size, domain_length=1.0):
+    axis = np.linspace(-domain_length / 2.0, domain_length / 2.0, grid_size, endpoint=False)
+    return np.meshgrid(axis, axis, axis, indexing='ij')
+
+
+def gauge_wave_analytic_state(X, Y, Z, t=0.0, amplitude=0.1, wavelength=1.0):
+    """Exact harmonic gauge wave (tests/initial_data.py:147-205)."""
+    shape = X.shape
+    phase = (2.0 * np.pi * (X - t)) / wavelength
+    H0 = 1.0 - amplitude * np.sin(phase)
+    lapse = np.sqrt(H0)
+    W = np.power(H0, -1.0 / 6.0)
+    gamma = np.zeros((3, 3) + shape)
+    gamma[0, 0] = H0 * W ** 2
+    gamma[1, 1] = W ** 2
+    gamma[2, 2] = W ** 2
+    Kij = np.zeros_like(gamma)
+    Kij[0, 0] = -(np.pi * amplitude / wavelength) * np.cos(phase) / np.sqrt(H0)
+    inv = invert_3x3_metric(gamma)
+    A = W ** 2 * traceless_part(Kij, gamma, inv)
+    K = W ** 2 * trace_tensor(Kij, inv)
+    dHdx = -(2.0 * np.pi * amplitude / wavelength) * np.cos(phase)
+    Gam = np.zeros((3,) + shape)
+    Gam[0] = (2.0 / 3.0) * H0 ** (-5.0 / 3.0) * dHdx
+    return BSSNVariables(gamma, W, A, K, Gam, lapse, np.zeros((3,) + shape))
+
+
+def periodic_gauge_wave_state(X, Y, Z, dx, amplitude=0.1, wavelength=1.0):
+    """Historical gauge-wave regression state (tests/initial_data.py:76-144)."""
+    shape = X.shape
+    phase = 2.0 * np.pi * X / wavelength
+    H = amplitude * np.sin(phase)
+    g_phys = np.zeros((3, 3) + shape)
+    g_phys[0, 0] = 1.0 - H
+    g_phys[1, 1] = 1.0
+    g_phys[2, 2] = 1.0
+    W = np.power(1.0 - H, -1.0 / 6.0)
+    gamma = np.zeros((3, 3) + shape)
+    gamma[0, 0] = np.power(1.0 - H, 2.0 / 3.0)
+    gamma[1, 1] = np.power(1.0 - H, -1.0 / 3.0)
+    gamma[2, 2] = np.power(1.0 - H, -1.0 / 3.0)
+    lapse = np.sqrt(1.0 - H)
+    derivs = np.stack([diff1_field(gamma, d + 2, dx) for d in range(3)])
+    inv = invert_3x3_metric(gamma)
+    c2 = christoffel_symbols_second_kind(inv, derivs)
+    Gam = np.einsum('mn...,imn...->i...', inv, c2)
+    Kij = np.zeros_like(g_phys)
+    Kij[0, 0] = -amplitude * np.pi * np.cos(phase) / (wavelength * lapse)
+    inv_phys = invert_3x3_metric(g_phys)
+    K = np.einsum('mn...,mn...->...', inv_phys, Kij)
+    A = W ** 2 * (Kij - g_phys * K / 3.0)
+    return BSSNVariables(gamma, W, A, K, Gam, lapse, np.zeros((3,) + shape))
+
+
+def brill_lindquist_conformal_factor(masses, positions, X, Y, Z):
+    """psi = 1 + sum_a m_a / (2 r_a) (utilities/bowen_york_solver.py:52-63)."""
+    psi = np.ones_like(X)
+    for m, p in zip(masses, positions):
+        r = np.sqrt((X - p[0]) ** 2 + (Y - p[1]) ** 2 + (Z - p[2]) ** 2)
+        psi = psi + m / (2.0 * r)
+    return psi
+
+
+def puncture_state(X, Y, Z, masses=(1.0,), positions=((0.0, 0.0, 0.0),),
+                   lapse_perturbation=None):
+    """Zero-momentum puncture data: W = psi^-2, alpha = W (pre-collapsed),
+    gamma_tilde = delta, A = K = Gamma = beta = 0
+    (demos/axisymmetric_blackhole_collision/axisymmetric_blackhole_collision.py:118-128).
+    ``lapse_perturbation`` = (eps, centre, width) adds eps*exp(-|x-c|^2/width^2).
+    """
+    psi = brill_lindquist_conformal_factor(masses, positions, X, Y, Z)
+    W = psi ** -2
+    shape = X.shape
+    gamma = np.zeros((3, 3) + shape)
+    for i in range(3):
+        gamma[i, i] = 1.0
+    lapse = W.copy()
+    if lapse_perturbation is not None:
+        eps, c, width = lapse_perturbation
+        r2 = (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2
+        lapse = lapse + eps * np.exp(-r2 / width ** 2)
+    z3 = np.zeros((3,) + shape)
+    return BSSNVariables(gamma, W, np.zeros((3, 3) + shape), np.zeros(shape), z3.copy(),
+                         lapse, z3.copy())
