@@ -1,0 +1,183 @@
+"""Throughput of the fused BSSN RK4 step on one MI355X.
+
+Workload (the single-puncture configuration): Brill-Lindquist puncture M = 1
+at the origin, zero momentum, 256^3 cell-centred grid on [-8, 8]^3, FP64,
+1+log slicing, Gamma-driver shift (eta = 2, g = 0.75), kappa = 0, KO
+nu = 0.1, Sommerfeld faces, dt = dx / 4.  Initial data is generated on the
+device (synthetic: closed-form psi = 1 + M / 2r, W = psi^-2, alpha = W).
+
+A step is one full RK4 step (4 fused stage launches + the Sommerfeld shell
+launches).  Reported: cell-updates/s (interior points x steps / time), the
+FP32-RHS variant, the HBM roofline fraction of the tile kernel from HIP-event
+timings of its launches, and the FP64 NumPy oracle timed on the host.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
+BYTES_PER_CELL = {1: 3 * 21 * 8, 2: 5 * 21 * 8, 3: 5 * 21 * 8, 4: 3 * 21 * 8}  # algorithmic, per stage
+
+
+def puncture_state(n, L=8.0, mass=1.0, device="cuda"):
+    import torch
+    dx = 2 * L / n
+    x = -L + (torch.arange(n, dtype=torch.float64, device=device) + 0.5) * dx
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    r = torch.sqrt(X * X + Y * Y + Z * Z)
+    W = (1.0 + mass / (2.0 * r)) ** -2
+    s = torch.zeros((21, n, n, n), dtype=torch.float64, device=device)
+    s[0] = 1.0
+    s[3] = 1.0
+    s[5] = 1.0
+    s[6] = W
+    s[17] = W
+    return s, dx, -L + 0.5 * dx
+
+
+def puncture_params(P, dx, x0):
+    return P(eta=2.0, kappa=0.0, nu=0.1, g=0.75, dx=dx, dt=0.25 * dx, zero_shift=0, gauge=1,
+             xl_bc=1, xr_bc=1, yl_bc=1, yr_bc=1, zl_bc=1, zr_bc=1, x_min=x0, y_min=x0, z_min=x0)
+
+
+def time_steps(J, state, params, steps, warmup, precision):
+    import torch
+    ev = J.Evolver(state, params, precision=precision)
+    ev.step(warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ev.step(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3, ev
+
+
+def tile_kernel_roofline(J, ev, precision, reps=3):
+    """Average tile-kernel launch duration per stage (HIP events on the launch
+    stream) and the algorithmic bytes those launches move."""
+    from jax_bssn_amd import _lib
+    lib = _lib.load()
+    cp, cg = ev._cp, ev._cg
+    prec = _lib.PREC_F64 if precision == "f64" else _lib.PREC_F32RHS
+    npts = int(lib.bssn_stage_points(ctypes.byref(cp), ctypes.byref(cg), 1))
+    u0 = ev.state
+    ustar, ua, ub = ev.work[0], ev.work[1], ev.work[2]
+    ua.copy_(u0)
+    total_bytes, total_s, per_stage = 0.0, 0.0, {}
+    st = ctypes.c_void_p(__import__("torch").cuda.current_stream().cuda_stream)
+    for s in (1, 2, 3, 4):
+        ms = ctypes.c_float(0.0)
+        us = u0 if s == 1 else ua
+        out = ub
+        _lib.check(lib.bssn_time_stage_part(ctypes.byref(cp), ctypes.byref(cg), prec, s, 1,
+                                            ctypes.c_void_p(us.data_ptr()), ctypes.c_void_p(u0.data_ptr()),
+                                            ctypes.c_void_p(ustar.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                            reps, ctypes.byref(ms), st))
+        per_stage[s] = ms.value
+        total_bytes += BYTES_PER_CELL[s] * npts
+        total_s += ms.value / 1e3
+    achieved = total_bytes / total_s
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "kernel": "k_stage_tile", "launch_ms_by_stage": per_stage,
+            "bytes_per_launch_avg": total_bytes / 4, "points_per_launch": npts}
+
+
+def cpu_baseline(n=48, steps=1):
+    """FP64 NumPy oracle on the host: same puncture configuration at n^3."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from oracle import bssn as OB
+    from oracle import initial_data as OI
+    L = 8.0
+    dx = 2 * L / n
+    x = -L + (np.arange(n) + 0.5) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    v = OI.puncture_state(X, Y, Z, masses=(1.0,), positions=((0.0, 0.0, 0.0),))
+    p = puncture_params(OB.BSSNParameters, dx, x[0])
+    with threadpool_limits(limits=1):
+        v = OB.rk4_step(v, p)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            v = OB.rk4_step(v, p)
+        dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK4 step(s) of the same puncture configuration at {n}^3 "
+                      f"(dx = {dx}), NumPy FP64 oracle, 1 thread, after 1 warm-up step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-f32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "halo-exchange path is design-only in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import jax_bssn_amd as J
+    torch.cuda.set_device(int(os.environ.get("LOCAL_RANK", 0)))
+    n = args.size
+    state, dx, x0 = puncture_state(n)
+    params = puncture_params(J.BSSNParameters, dx, x0)
+    cells = n ** 3
+
+    t64, ev64 = time_steps(J, state.clone(), params, args.steps, args.warmup, "f64")
+    rate64 = cells * args.steps / t64
+    roof = tile_kernel_roofline(J, ev64, "f64")
+    del ev64
+    extra = {}
+    if not args.no_f32:
+        t32, ev32 = time_steps(J, state.clone(), params, args.steps, args.warmup, "f32rhs")
+        rate32 = cells * args.steps / t32
+        roof32 = tile_kernel_roofline(J, ev32, "f32rhs")
+        del ev32
+        extra["f32rhs"] = {"value": rate32, "ms_per_step": 1e3 * t32 / args.steps,
+                           "speedup_vs_f64": rate32 / rate64, "tile_kernel_hbm_frac": roof32["frac"]}
+    torch.cuda.synchronize()
+    line = {
+        "metric": "BSSN RK4 cell-updates/s at 256^3 FP64 (single puncture)",
+        "value": rate64,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t64 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (closed-form Brill-Lindquist puncture generated on the device)",
+        "config": {"workload": f"single puncture M=1, {n}^3 on [-8,8]^3, 1+log + Gamma driver, "
+                               "Sommerfeld faces, KO nu=0.1, dt=dx/4, FP64 RK4",
+                   "grid": [n, n, n], "global_batch": 1, "parallelism": "single GPU"},
+        "roofline": roof,
+        "adjoint_forward_ratio": None,
+    }
+    line.update(extra)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

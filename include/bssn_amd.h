@@ -1,0 +1,139 @@
+/*
+ * bssn_amd.h -- C ABI of the MI355X (gfx950) BSSN evolution library.
+ *
+ * Plain C: device pointers, sizes and a HIP stream passed as void*.  Every
+ * entry point is asynchronous on the given stream (NULL = the default
+ * stream), retains nothing after it returns, never allocates unless stated,
+ * and returns 0 on success or a negative status; bssn_last_error_string()
+ * describes the last failure of the calling thread.
+ *
+ * State layout ("packed state"): 21 FP64 component arrays of nx*ny*nz
+ * points each, back to back (component c at state + c*nx*ny*nz), every
+ * array C-order with z fastest -- the reference layout of each field
+ * (reference docs/bssn.rst:11-12, bssn/variables.py:8-16) with the symmetric
+ * tensors stored once:
+ *
+ *    0..5   conformal_metric  xx xy xz yy yz zz
+ *    6      conformal_factor  W
+ *    7..12  traceless_K       xx xy xz yy yz zz
+ *    13     trace_K
+ *    14..16 conformal_connection x y z
+ *    17     lapse
+ *    18..20 shift x y z
+ *
+ * Arrays are the whole periodic domain with no ghost points: periodic axes
+ * wrap (the reference's jnp.roll), Sommerfeld axes use one-sided closures,
+ * exactly like the reference unigrid path.  Guard cells / owned ranges of
+ * the reference's tiled path are described by bssn_params.
+ */
+#ifndef BSSN_AMD_H
+#define BSSN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSSN_NCOMP 21
+#define BSSN_OK 0
+#define BSSN_ERR_INVALID (-1)   /* invalid configuration (Python: ValueError) */
+#define BSSN_ERR_HIP (-2)       /* HIP runtime failure   (Python: RuntimeError) */
+#define BSSN_ERR_UNSUPPORTED (-3)
+
+/* Mirrors reference BSSNParameters (bssn/variables.py:19-46) field by field.
+ * owned_shape[d] < 0 means "None" (the whole array extent), reduced_axes[0]
+ * < 0 means "None" (legacy singleton inference). */
+typedef struct bssn_params {
+    double eta, kappa, nu, g, dx, dt;
+    double x_min, y_min, z_min;
+    double mad_q;
+    int32_t zero_shift;      /* 1: the shift RHS is zero                */
+    int32_t gauge;           /* 0 harmonic, 1 one-plus-log              */
+    int32_t bc[6];           /* xl xr yl yr zl zr: 0 periodic, 1 Sommerfeld */
+    int32_t guard_cells;
+    int32_t owned_shape[3];
+    int32_t owned_faces[6];
+    int32_t reduced_axes[3];
+} bssn_params;
+
+/* Array extents of one evolution. */
+typedef struct bssn_grid {
+    int32_t nx, ny, nz;
+} bssn_grid;
+
+/* Arithmetic of the RHS: BSSN_PREC_F64 everything in FP64; BSSN_PREC_F32RHS
+ * derivatives and RHS algebra in FP32 while the RK4 state, accumulators,
+ * projections and reductions stay FP64. */
+#define BSSN_PREC_F64 0
+#define BSSN_PREC_F32RHS 1
+
+const char *bssn_last_error_string(void);
+
+/* Library version string ("bssn_amd <semver> gfx950"). */
+const char *bssn_version(void);
+
+/* Replaces reference evolution/time_evolve.py:41-57 compute_bssn_rhs.
+ * rhs[21*n] <- RHS(state[21*n]); state and rhs must not alias. */
+int bssn_rhs(const bssn_params *p, const bssn_grid *grid, int precision,
+             const double *state, double *rhs, void *stream);
+
+/* Replaces reference time_evolve.py:134-141 enforce_algebraic_constraints:
+ * out <- P(in) pointwise (det = 1 rescale, then trace-free A).  in == out
+ * is allowed. */
+int bssn_enforce_constraints(const bssn_grid *grid, const double *in,
+                             double *out, void *stream);
+
+/* One fused RK4 stage (reference time_evolve.py:144-217, one of the four
+ * RHS evaluations and the combination that follows it):
+ *   k = RHS(u_s);
+ *   stage 1: ustar  = u0 + dt/6 k;   u_next = P(u0 + dt/2 k)
+ *   stage 2: ustar += dt/3 k;        u_next = P(u0 + dt/2 k)
+ *   stage 3: ustar += dt/3 k;        u_next = P(u0 + dt k)
+ *   stage 4:                         u_next = P(ustar + dt/6 k)
+ *                                    (P(P(...)) when reproject != 0)
+ * u_next must alias neither u_s nor u0 (except stage 4, where u_next ==
+ * u0 is allowed); u0 is unused in stage 4; ustar is unused in stage 4's
+ * write set. */
+int bssn_rk4_stage(const bssn_params *p, const bssn_grid *grid, int precision,
+                   int stage, const double *u_s, const double *u0,
+                   double *ustar, double *u_next, int reproject, void *stream);
+
+/* Replaces reference time_evolve.py:144-217 rk4_step applied nsteps times:
+ * state <- rk4_step^nsteps(state), in place.  work must hold 3*21*n
+ * doubles (caller-owned; no allocation).  Identical to calling rk4_step
+ * nsteps times (the projection at the start of each step included). */
+int bssn_evolve(const bssn_params *p, const bssn_grid *grid, int precision,
+                int nsteps, double *state, double *work, void *stream);
+
+/* Replaces reference bssn/constraints.py:387-412 compute_all_constraints:
+ * out[9*n] = H, M_x, M_y, M_z, det-1, tr A, G^x, G^y, G^z (each n points). */
+int bssn_constraints(const bssn_params *p, const bssn_grid *grid,
+                     const double *state, double *out, void *stream);
+
+/* Replaces reference diagnostics/constraints.py:9-36 compute_constraint_norms
+ * over the constraint fields bssn_constraints writes.
+ * norms[10] = L2 and Linf of (H, M, det, trA, Gamma) in the order
+ * hamiltonian_l2, momentum_l2, det_gamma_l2, trace_A_l2, gamma_l2,
+ * hamiltonian_linf, momentum_linf, det_gamma_linf, trace_A_linf, gamma_linf
+ * (device pointer).  partial must hold bssn_norm_partial_size(grid) doubles. */
+int64_t bssn_norm_partial_size(const bssn_grid *grid);
+int bssn_constraint_norms(const bssn_grid *grid, const double *constraints,
+                          double *partial, double *norms, void *stream);
+
+/* Measurement helper (no reference counterpart): launches part 1 (tile
+ * kernel) or part 2 (Sommerfeld shell kernel) of RK4 stage `stage` `reps`
+ * times between two hipEvents recorded on `stream`, waits for them and
+ * stores the mean duration per launch in *ms.  Buffers as bssn_rk4_stage. */
+int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precision, int stage, int part,
+                         const double *u_s, const double *u0, double *ustar, double *u_next, int reps,
+                         float *ms, void *stream);
+
+/* Number of grid points part 1 (tile kernel) or part 2 (shell kernel) of a
+ * stage updates (negative status on invalid input). */
+int64_t bssn_stage_points(const bssn_params *p, const bssn_grid *grid, int part);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSSN_AMD_H */

@@ -1,0 +1,83 @@
+"""Constraint fields and norms (reference bssn/constraints.py:262-412,
+diagnostics/constraints.py:9-36), computed by the HIP constraint kernel and a
+two-pass wavefront reduction."""
+
+import ctypes
+from typing import NamedTuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .time_evolve import _like, _ptr, stream_handle
+from .variables import pack
+
+
+class ConstraintViolations(NamedTuple):
+    hamiltonian: object
+    momentum: object
+    det_gamma: object
+    trace_A: object
+    gamma_condition: object
+
+
+NORM_NAMES = ("hamiltonian_l2", "momentum_l2", "det_gamma_l2", "trace_A_l2", "gamma_l2",
+              "hamiltonian_linf", "momentum_linf", "det_gamma_linf", "trace_A_linf", "gamma_linf")
+
+
+def constraints_packed(state, params, out=None):
+    """(9, nx, ny, nz): H, M_x, M_y, M_z, det-1, tr A, G^x, G^y, G^z."""
+    lib = _lib.load()
+    if out is None:
+        out = torch.empty((9,) + tuple(state.shape[1:]), dtype=torch.float64, device=state.device)
+    _lib.check(lib.bssn_constraints(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(state.shape[1:])),
+                                    _ptr(state), _ptr(out), stream_handle()))
+    return out
+
+
+def norms_packed(cfields):
+    """Device tensor of the 10 norms of a (9, nx, ny, nz) constraint tensor."""
+    lib = _lib.load()
+    grid = _lib.c_grid(cfields.shape[1:])
+    partial = torch.empty(int(lib.bssn_norm_partial_size(ctypes.byref(grid))), dtype=torch.float64,
+                          device=cfields.device)
+    norms = torch.empty(10, dtype=torch.float64, device=cfields.device)
+    _lib.check(lib.bssn_constraint_norms(ctypes.byref(grid), _ptr(cfields), _ptr(partial), _ptr(norms),
+                                         stream_handle()))
+    return norms
+
+
+def _split(cf, like):
+    to = (lambda t: t.cpu().numpy()) if isinstance(like, np.ndarray) else (lambda t: t.to(like.device))
+    return ConstraintViolations(to(cf[0].clone()), to(cf[1:4].clone()), to(cf[4].clone()),
+                                to(cf[5].clone()), to(cf[6:9].clone()))
+
+
+def compute_all_constraints(vars, params):
+    """Reference bssn/constraints.py:387-412."""
+    return _split(constraints_packed(pack(vars), params), _like(vars))
+
+
+def compute_hamiltonian_constraint(vars, params):
+    return compute_all_constraints(vars, params).hamiltonian
+
+
+def compute_constraint_norms(violations):
+    """Reference diagnostics/constraints.py:9-36: dict of L2 / Linf norms."""
+    from .variables import device
+    dev = device()
+    h = torch.as_tensor(np.asarray(violations.hamiltonian) if not isinstance(violations.hamiltonian, torch.Tensor)
+                        else violations.hamiltonian, dtype=torch.float64).to(dev)
+    shape = tuple(h.shape)
+    cf = torch.empty((9,) + shape, dtype=torch.float64, device=dev)
+
+    def put(sl, x):
+        t = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x))
+        cf[sl].copy_(t.to(dev, torch.float64).reshape(cf[sl].shape))
+    put(0, violations.hamiltonian)
+    put(slice(1, 4), violations.momentum)
+    put(4, violations.det_gamma)
+    put(5, violations.trace_A)
+    put(slice(6, 9), violations.gamma_condition)
+    vals = norms_packed(cf).cpu().tolist()
+    return dict(zip(NORM_NAMES, vals))

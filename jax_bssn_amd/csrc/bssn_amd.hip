@@ -1,0 +1,761 @@
+// MI355X (gfx950) BSSN evolution: fused RHS + RK4 stage kernels, ghost-free
+// periodic wrap / Sommerfeld closure shells, algebraic projection,
+// constraints with wavefront reductions, and the C ABI of include/bssn_amd.h.
+//
+// One RK4 stage is one launch of the tile kernel (all points whose stencils
+// are interior) plus, when an axis has Sommerfeld faces, one launch of the
+// shell kernel over the three-point closure shells next to those faces.
+// Both evaluate the same templated per-point RHS (rhs_point.h) and finish
+// with the same in-register stage update and projection.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "bssn_amd.h"
+#include "bssn_common.h"
+#include "rhs_point.h"
+#include "stencil_providers.h"
+
+namespace bssn {
+
+// ------------------------------------------------------------ projection
+
+// Reference time_evolve.py:94-131: gamma <- gamma det^(-1/3); then
+// A <- A - 1/3 gamma (gamma^ij A_ij) with the inverse of the rescaled metric.
+__device__ __forceinline__ void project(double (&g)[6], double (&A)[6]) {
+  const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+                     g[2] * (g[1] * g[4] - g[3] * g[2]);
+  const double s = pow(det, -1.0 / 3.0);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) g[c] *= s;
+  double ig[6];
+  inverse_metric(g, ig);
+  const double tr = contract(ig, A);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) A[c] -= (1.0 / 3.0) * g[c] * tr;
+}
+
+// ------------------------------------------------------------ stage update
+
+// Receives each component's RHS as soon as it is final and applies the
+// stage update of bssn_rk4_stage (include/bssn_amd.h) right away; the
+// metric and A are held until done() so they can be projected together.
+struct Emitter {
+  int64_t n, idx;
+  double dt;
+  const double *u0;
+  double *ustar, *unext, *rhs;
+  int mode, reproject;
+  bool active;
+  double gn[6], An[6];
+
+  __device__ Emitter(const Ctx &C, const StageOut &O, int64_t i, bool act)
+      : n(C.ncell), idx(i), dt(C.dt), u0(O.u0), ustar(O.ustar), unext(O.unext), rhs(O.rhs), mode(O.mode),
+        reproject(O.reproject), active(act) {}
+
+  __device__ void put(int c, double k) {
+    if (!active) return;
+    const int64_t at = c * n + idx;
+    if (mode == 0) {
+      rhs[at] = k;
+      return;
+    }
+    double un;
+    if (mode == 1) {
+      const double v0 = u0[at];
+      ustar[at] = v0 + (dt / 6.0) * k;
+      un = v0 + (0.5 * dt) * k;
+    } else if (mode == 4) {
+      un = ustar[at] + (dt / 6.0) * k;
+    } else {
+      const double v0 = u0[at];
+      ustar[at] = ustar[at] + (dt / 3.0) * k;
+      un = v0 + (mode == 2 ? 0.5 * dt : dt) * k;
+    }
+    if (c <= GZZ) {
+      gn[c] = un;
+    } else if (c >= AXX && c <= AZZ) {
+      An[c - AXX] = un;
+    } else {
+      unext[at] = un;
+    }
+  }
+
+  __device__ void done() {
+    if (!active || mode == 0) return;
+    project(gn, An);
+    if (mode == 4 && reproject) project(gn, An);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      unext[(GXX + c) * n + idx] = gn[c];
+      unext[(AXX + c) * n + idx] = An[c];
+    }
+  }
+};
+
+// ----------------------------------------------------------- shell logic
+
+__device__ __forceinline__ bool in_shell(const Ctx &C, int x, int y, int z) {
+  const int p[3] = {x, y, z};
+  bool s = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const Axis &A = C.ax[a];
+    const int lo = p[a] - A.start, hi = A.end - 1 - p[a];
+    s |= (lo >= 0 && lo < A.shell_lo) || (hi >= 0 && hi < A.shell_hi);
+  }
+  return s;
+}
+
+__device__ __forceinline__ bool sommerfeld_point(const Ctx &C, int x, int y, int z) {
+  const int p[3] = {x, y, z};
+  bool face = false, owned = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const Axis &A = C.ax[a];
+    face |= (A.somm_lo && p[a] == A.start) || (A.somm_hi && p[a] == A.end - 1);
+    owned &= (p[a] >= A.start && p[a] < A.end);
+  }
+  return face && owned;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so tiles sharing halos share an L2.
+__device__ __forceinline__ int xcd_remap(int i, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X;
+  const int x = i % X, l = i / X;
+  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
+__device__ __forceinline__ void load_point(const Ctx &C, const double *__restrict__ u, int64_t idx,
+                                           double (&v)[NCOMP]) {
+#pragma unroll
+  for (int c = 0; c < NCOMP; ++c) v[c] = u[c * C.ncell + idx];
+}
+
+// ------------------------------------------------------------ tile kernel
+
+template <typename T, int TX, int TY, int TZ, bool MAD>
+__global__ void __launch_bounds__(TX *TY *TZ)
+    k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, int remap) {
+  using P = TileProvider<T, TX, TY, TZ, MAD>;
+  __shared__ T smem[P::LDS_ELEMS];
+  const int nbx = (C.ax[0].n + TX - 1) / TX;
+  const int nby = (C.ax[1].n + TY - 1) / TY;
+  const int nbz = (C.ax[2].n + TZ - 1) / TZ;
+  const int ntiles = nbx * nby * nbz;
+  const int bid = remap ? xcd_remap((int)blockIdx.x, ntiles) : (int)blockIdx.x;
+  const int bz = bid % nbz;
+  const int bt = bid / nbz;
+  const int by = bt % nby;
+  const int bx = bt / nby;
+  const int tid = threadIdx.x;
+  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
+  const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
+  const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;
+  const int yc = y < C.ax[1].n ? y : C.ax[1].n - 1;
+  const int zc = z < C.ax[2].n ? z : C.ax[2].n - 1;
+  const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
+
+  P pv;
+  pv.setup(C, us, smem, bx * TX, by * TY, bz * TZ, tx, ty, tz);
+  double u[NCOMP];
+  load_point(C, us, idx, u);
+  Emitter out(C, O, idx, inside && !(C.any_shell && in_shell(C, x, y, z)));
+  bssn_rhs_point<T>(pv, C, u, out);
+}
+
+// ----------------------------------------------------------- shell kernel
+
+struct Slabs {
+  int count;
+  int lo[6][3], hi[6][3];
+  int64_t first[7];  // prefix sums of slab sizes
+};
+
+__device__ __forceinline__ bool in_slab(const Slabs &S, int s, int x, int y, int z) {
+  return x >= S.lo[s][0] && x < S.hi[s][0] && y >= S.lo[s][1] && y < S.hi[s][1] && z >= S.lo[s][2] &&
+         z < S.hi[s][2];
+}
+
+// Sommerfeld RHS of component c (reference evolution/boundaries.py:29-92).
+template <typename T>
+__device__ T sommerfeld_rhs(PointProvider<T> &pv, const Ctx &C, int c, double uc, int x, int y, int z) {
+  pv.stage_comp(c);
+  const T d0 = pv.d1(0), d1 = pv.d1(1), d2 = pv.d1(2);
+  const T X = (T)(C.xmin[0] + C.dx * (double)x);
+  const T Y = (T)(C.xmin[1] + C.dx * (double)y);
+  const T Z = (T)(C.xmin[2] + C.dx * (double)z);
+  const T r = sqrt(X * X + Y * Y + Z * Z);
+  const T rs = r > T(0) ? r : T(1);
+  const bool unit = (c == GXX || c == GYY || c == GZZ || c == CW || c == ALPHA);
+  const T u0 = unit ? T(1) : T(0);
+  const T radial = (X * d0 + Y * d1 + Z * d2) / rs;
+  return -(radial + ((T)uc - u0) / rs);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_stage_shell(Ctx C, const double *__restrict__ us, StageOut O, Slabs S) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= S.first[S.count]) return;
+  int s = 0;
+  while (s + 1 < S.count && gid >= S.first[s + 1]) ++s;
+  const int64_t l = gid - S.first[s];
+  const int ez = S.hi[s][2] - S.lo[s][2], ey = S.hi[s][1] - S.lo[s][1];
+  const int z = S.lo[s][2] + (int)(l % ez);
+  const int y = S.lo[s][1] + (int)((l / ez) % ey);
+  const int x = S.lo[s][0] + (int)(l / ((int64_t)ez * ey));
+  for (int t = 0; t < s; ++t)
+    if (in_slab(S, t, x, y, z)) return;
+
+  PointProvider<T> pv;
+  pv.setup(C, us, x, y, z);
+  double u[NCOMP];
+  load_point(C, us, pv.idx, u);
+  Emitter out(C, O, pv.idx, true);
+  if (sommerfeld_point(C, x, y, z)) {
+    // the reference replaces the whole RHS on active faces
+    for (int c = 0; c < NCOMP; ++c) out.put(c, (double)sommerfeld_rhs<T>(pv, C, c, u[c], x, y, z));
+    out.done();
+  } else {
+    bssn_rhs_point<T>(pv, C, u, out);
+  }
+}
+
+// ------------------------------------------------------ projection kernel
+
+__global__ void __launch_bounds__(256) k_project(int64_t n, const double *__restrict__ in, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double g[6], A[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    g[c] = in[(GXX + c) * n + i];
+    A[c] = in[(AXX + c) * n + i];
+  }
+  project(g, A);
+  if (in != out) {
+    for (int c = 0; c < NCOMP; ++c)
+      if (c < GXX || (c > GZZ && c < AXX) || c > AZZ) out[c * n + i] = in[c * n + i];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    out[(GXX + c) * n + i] = g[c];
+    out[(AXX + c) * n + i] = A[c];
+  }
+}
+
+// ------------------------------------------------------ constraint kernel
+
+// Reference bssn/constraints.py:22-97 (momentum), 271-303 (Hamiltonian),
+// 318-350 (det, trace), 353-384 (Gamma constraint).
+__global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__restrict__ us, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.ncell) return;
+  const int z = (int)(i % C.ax[2].n);
+  const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
+  const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
+  PointProvider<double> pv;
+  pv.setup(C, us, x, y, z);
+  double u[NCOMP];
+  load_point(C, us, pv.idx, u);
+  double g[6], A[6], ig[6];
+  for (int c = 0; c < 6; ++c) {
+    g[c] = u[GXX + c];
+    A[c] = u[AXX + c];
+  }
+  inverse_metric(g, ig);
+  const double W = u[CW], W2 = W * W, K = u[CK];
+  double R[6] = {0, 0, 0, 0, 0, 0};
+  double dg[6][3], dW[3], dK[3];
+  // W: second derivatives
+  pv.stage_comp(CW);
+  for (int j = 0; j < 3; ++j) dW[j] = pv.d1(j);
+  {
+    double s[6];
+    second_derivatives(pv, s);
+    const double lap = contract(ig, s);
+    for (int c = 0; c < 6; ++c) R[c] += W * (s[c] + g[c] * lap);
+  }
+  pv.stage_comp(CK);
+  for (int j = 0; j < 3; ++j) dK[j] = pv.d1(j);
+  for (int m = 0; m < 3; ++m) {
+    pv.stage_comp(GAM + m);
+    double dG[3];
+    for (int j = 0; j < 3; ++j) dG[j] = pv.d1(j);
+    for (int c = 0; c < 6; ++c) {
+      const int a = SYM_I[c], b = SYM_J[c];
+      R[c] += W2 * 0.5 * (g[sym(m, a)] * dG[b] + g[sym(m, b)] * dG[a]);
+    }
+  }
+  for (int c = 0; c < 6; ++c) {
+    pv.stage_comp(GXX + c);
+    for (int d = 0; d < 3; ++d) dg[c][d] = pv.d1(d);
+    double s[6];
+    second_derivatives(pv, s);
+    R[c] += -0.5 * W2 * contract(ig, s);
+  }
+  double C1[3][6], C2[3][6];
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 6; ++c) {
+      const int j = SYM_I[c], k = SYM_J[c];
+      C1[a][c] = 0.5 * (dg[sym(a, j)][k] + dg[sym(a, k)][j] - dg[c][a]);
+    }
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 6; ++c)
+      C2[a][c] = ig[sym(a, 0)] * C1[0][c] + ig[sym(a, 1)] * C1[1][c] + ig[sym(a, 2)] * C1[2][c];
+  const double G[3] = {u[GAM], u[GAM + 1], u[GAM + 2]};
+  {
+    const double gradW2 = contract_vec(ig, dW, dW);
+    double conW = 0.0;
+    for (int k = 0; k < 3; ++k) conW += contract(ig, C2[k]) * dW[k];
+    for (int c = 0; c < 6; ++c) {
+      const int a = SYM_I[c], b = SYM_J[c];
+      double t3 = 0, t4 = 0, t5 = 0;
+      for (int m = 0; m < 3; ++m) t3 += G[m] * (C1[a][sym(b, m)] + C1[b][sym(a, m)]);
+      for (int m = 0; m < 3; ++m)
+        for (int n2 = 0; n2 < 3; ++n2) {
+          const double gmn = ig[sym(m, n2)];
+          double acc4 = 0, acc5 = 0;
+          for (int k = 0; k < 3; ++k) {
+            acc4 += C2[k][sym(m, a)] * C1[b][sym(k, n2)] + C2[k][sym(m, b)] * C1[a][sym(k, n2)];
+            acc5 += C2[k][sym(a, m)] * C1[k][sym(b, n2)];
+          }
+          t4 += gmn * acc4;
+          t5 += gmn * acc5;
+        }
+      const double chW = C2[0][c] * dW[0] + C2[1][c] * dW[1] + C2[2][c] * dW[2];
+      R[c] += W2 * (0.5 * t3 + t4 + t5) - W * (chW + g[c] * conW) - 2.0 * g[c] * gradW2;
+    }
+  }
+  double Aup[6];
+  for (int c = 0; c < 6; ++c) {
+    const int a = SYM_I[c], b = SYM_J[c];
+    double acc = 0;
+    for (int k = 0; k < 3; ++k)
+      for (int l2 = 0; l2 < 3; ++l2) acc += ig[sym(a, k)] * ig[sym(b, l2)] * A[sym(k, l2)];
+    Aup[c] = acc;
+  }
+  const int64_t n = C.ncell;
+  out[0 * n + i] = contract(ig, R) + 2.0 / 3.0 * K * K - contract(A, Aup);
+
+  // momentum constraint
+  const double Wf = W > 1.0e-12 ? W : 1.0e-12;
+  auto b_at = [&](int a, int o, int r, int col) -> double {
+    // B_r^col = gamma^{col k} A_{r k} at the point offset by o along axis a
+    double gg[6], AA[3], ii[6];
+    int q[3] = {x, y, z};
+    q[a] = wrap_index(q[a] + o, C.ax[a].n);
+    const int64_t j = (int64_t)q[0] * C.ax[0].stride + (int64_t)q[1] * C.ax[1].stride + q[2];
+    for (int c = 0; c < 6; ++c) gg[c] = us[(GXX + c) * n + j];
+    for (int k = 0; k < 3; ++k) AA[k] = us[(AXX + sym(r, k)) * n + j];
+    inverse_metric(gg, ii);
+    return ii[sym(col, 0)] * AA[0] + ii[sym(col, 1)] * AA[1] + ii[sym(col, 2)] * AA[2];
+  };
+  double dA[6][3];
+  for (int c = 0; c < 6; ++c) {
+    pv.stage_comp(AXX + c);
+    for (int d = 0; d < 3; ++d) dA[c][d] = pv.d1(d);
+  }
+  for (int r = 0; r < 3; ++r) {
+    double first = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      if (C.ax[j].reduced) continue;
+      Row row = row_d1(C.ax[j], j == 0 ? x : (j == 1 ? y : z), C.inv_dx, false);
+      double s = 0.0;
+      for (int k = 0; k < row.n; ++k) s += row.c[k] * b_at(j, row.off[k], r, j);
+      first += s;
+    }
+    double Bw = 0.0;  // B_r^j d_j W
+    for (int j = 0; j < 3; ++j) {
+      double B = ig[sym(j, 0)] * A[sym(r, 0)] + ig[sym(j, 1)] * A[sym(r, 1)] + ig[sym(j, 2)] * A[sym(r, 2)];
+      Bw += B * dW[j];
+    }
+    double second = 0.0;
+    for (int c = 0; c < 6; ++c) second += (SYM_I[c] == SYM_J[c] ? 1.0 : 2.0) * ig[c] * dA[c][r];
+    out[(1 + r) * n + i] = first - 0.5 * second - 3.0 * Bw / Wf - 2.0 / 3.0 * dK[r];
+  }
+  const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+                     g[2] * (g[1] * g[4] - g[3] * g[2]);
+  out[4 * n + i] = det - 1.0;
+  out[5 * n + i] = contract(ig, A);
+  for (int r = 0; r < 3; ++r) out[(6 + r) * n + i] = G[r] - contract(ig, C2[r]);
+}
+
+// ------------------------------------------------------------- reductions
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS_MAX = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// partial[b*10 + q]: q = 0..4 sums of squares (H, M, det, trA, Gamma), 5..9 max |.|
+__global__ void __launch_bounds__(RED_THREADS) k_norm_partial(int64_t n, const double *__restrict__ cf,
+                                                               double *partial) {
+  double acc[10];
+  for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+  const int group[9] = {0, 1, 1, 1, 2, 3, 4, 4, 4};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+      const double v = cf[f * n + i];
+      acc[group[f]] += v * v;
+      acc[5 + group[f]] = fmax(acc[5 + group[f]], fabs(v));
+    }
+  }
+  __shared__ double red[RED_THREADS / 64][10];
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    const double v = q < 5 ? wave_sum(acc[q]) : wave_max(acc[q]);
+    if (lane == 0) red[wid][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    const int q = threadIdx.x;
+    double v = red[0][q];
+    for (int w = 1; w < RED_THREADS / 64; ++w) v = q < 5 ? v + red[w][q] : fmax(v, red[w][q]);
+    partial[blockIdx.x * 10 + q] = v;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_norm_final(int nblocks, int64_t n, const double *__restrict__ partial,
+                                                   double *norms) {
+  const int q = threadIdx.x;
+  if (q >= 10) return;
+  double v = 0.0;
+  for (int b = 0; b < nblocks; ++b) v = q < 5 ? v + partial[b * 10 + q] : fmax(v, partial[b * 10 + q]);
+  if (q < 5) {
+    const double count = (q == 1 || q == 4) ? 3.0 * (double)n : (double)n;
+    v = sqrt(v / count);
+  }
+  norms[q] = v;
+}
+
+}  // namespace bssn
+
+// =================================================================== host
+
+using namespace bssn;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+  if (e != hipSuccess) return fail(BSSN_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  return BSSN_OK;
+}
+
+// Reference evolution/spatial_derivatives.py:18-41 (_boundary_data),
+// evolution/boundaries.py:44-72 (mask) and the validators of
+// parallel/initialization.py:31-37 / parallel/layout.py:92-108.
+int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
+  if (!p || !grid || !C) return fail(BSSN_ERR_INVALID, "null argument");
+  const int n[3] = {grid->nx, grid->ny, grid->nz};
+  for (int d = 0; d < 3; ++d)
+    if (n[d] < 1) return fail(BSSN_ERR_INVALID, "grid extent %d along axis %d must be positive", n[d], d);
+  const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+  if (ncell >= (int64_t(1) << 31)) return fail(BSSN_ERR_INVALID, "grid of %lld points exceeds 2^31", (long long)ncell);
+  if (!(p->dx > 0.0) || !std::isfinite(p->dx)) return fail(BSSN_ERR_INVALID, "dx must be finite and positive");
+  if (p->guard_cells < 0) return fail(BSSN_ERR_INVALID, "guard_cells must be non-negative");
+  std::memset(C, 0, sizeof(Ctx));
+  const bool owned_given = p->owned_shape[0] >= 0;
+  int any_shell = 0;
+  for (int d = 0; d < 3; ++d) {
+    Axis &A = C->ax[d];
+    const int lc = p->bc[2 * d], rc = p->bc[2 * d + 1];
+    if ((lc != 0 && lc != 1) || (rc != 0 && rc != 1))
+      return fail(BSSN_ERR_INVALID, "boundary codes must be 0 (periodic) or 1 (Sommerfeld)");
+    const int count = owned_given ? p->owned_shape[d] : n[d];
+    A.n = n[d];
+    A.start = p->guard_cells;
+    A.end = A.start + count;
+    if (count < 1 || A.end > n[d])
+      return fail(BSSN_ERR_INVALID, "owned range [%d, %d) does not fit axis %d of extent %d", A.start, A.end, d, n[d]);
+    const bool global_somm = lc == 1 || rc == 1;
+    A.q = global_somm ? 1.0 : p->mad_q;
+    A.lbc = (lc == 1 && p->owned_faces[2 * d]) ? 1 : 0;
+    A.rbc = (rc == 1 && p->owned_faces[2 * d + 1]) ? 1 : 0;
+    if (p->reduced_axes[0] >= 0)
+      A.reduced = p->reduced_axes[d] ? 1 : 0;
+    else
+      A.reduced = (owned_given && count == 1) ? 1 : 0;
+    A.somm_lo = (A.lbc && (!owned_given || count > 1)) ? 1 : 0;
+    A.somm_hi = (A.rbc && (!owned_given || count > 1)) ? 1 : 0;
+    if (A.reduced && global_somm)
+      return fail(BSSN_ERR_INVALID, "axis %d is reduced and cannot carry a Sommerfeld face", d);
+    if ((A.lbc || A.rbc) && count < 8)
+      return fail(BSSN_ERR_INVALID, "Sommerfeld closures on axis %d need at least 8 owned points (have %d)", d, count);
+    A.shell_lo = A.lbc ? 3 : 0;
+    A.shell_hi = A.rbc ? 3 : 0;
+    any_shell |= A.lbc | A.rbc;
+  }
+  C->ax[0].stride = n[1] * n[2];
+  C->ax[1].stride = n[2];
+  C->ax[2].stride = 1;
+  C->ncell = ncell;
+  C->dx = p->dx;
+  C->inv_dx = 1.0 / p->dx;
+  C->inv_dx2 = 1.0 / (p->dx * p->dx);
+  C->ko = p->nu / 64.0 * std::pow(p->dx, 5) / std::pow(p->dx, 6);
+  C->eta = p->eta;
+  C->g = p->g;
+  C->kappa = p->kappa;
+  C->dt = p->dt;
+  C->xmin[0] = p->x_min;
+  C->xmin[1] = p->y_min;
+  C->xmin[2] = p->z_min;
+  C->gauge = p->gauge;
+  C->zero_shift = p->zero_shift;
+  C->any_shell = any_shell;
+  if (p->gauge != 0 && p->gauge != 1) return fail(BSSN_ERR_INVALID, "gauge must be 0 (harmonic) or 1 (1+log)");
+  if (p->kappa != 0.0)
+    return fail(BSSN_ERR_UNSUPPORTED, "kappa != 0 (constraint damping) is not implemented yet");
+  return BSSN_OK;
+}
+
+Slabs make_slabs(const Ctx &C) {
+  Slabs S;
+  std::memset(&S, 0, sizeof S);
+  int64_t total = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int side = 0; side < 2; ++side) {
+      const Axis &A = C.ax[a];
+      const int w = side == 0 ? A.shell_lo : A.shell_hi;
+      if (!w) continue;
+      const int s = S.count++;
+      for (int b = 0; b < 3; ++b) {
+        S.lo[s][b] = 0;
+        S.hi[s][b] = C.ax[b].n;
+      }
+      if (side == 0) {
+        S.lo[s][a] = A.start;
+        S.hi[s][a] = A.start + w;
+      } else {
+        S.lo[s][a] = A.end - w;
+        S.hi[s][a] = A.end;
+      }
+      S.first[s] = total;
+      total += (int64_t)(S.hi[s][0] - S.lo[s][0]) * (S.hi[s][1] - S.lo[s][1]) * (S.hi[s][2] - S.lo[s][2]);
+    }
+  }
+  S.first[S.count] = total;
+  return S;
+}
+
+constexpr int TX = 4, TY = 8, TZ = 8;
+
+template <typename T, bool MAD>
+int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
+  const int nbx = (C.ax[0].n + TX - 1) / TX, nby = (C.ax[1].n + TY - 1) / TY, nbz = (C.ax[2].n + TZ - 1) / TZ;
+  const int ntiles = nbx * nby * nbz;
+  int rc = BSSN_OK;
+  if (parts & 1) {
+    hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us, O, 1);
+    rc = check_hip(hipGetLastError(), "tile kernel launch");
+    if (rc) return rc;
+  }
+  if ((parts & 2) && C.any_shell) {
+    Slabs S = make_slabs(C);
+    const int64_t total = S.first[S.count];
+    if (total > 0) {
+      const int64_t blocks = (total + 255) / 256;
+      hipLaunchKernelGGL((k_stage_shell<T>), dim3((unsigned)blocks), dim3(256), 0, st, C, us, O, S);
+      rc = check_hip(hipGetLastError(), "shell kernel launch");
+    }
+  }
+  return rc;
+}
+
+int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &O, hipStream_t st, int parts = 3) {
+  const bool mad = C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0;
+  if (precision == BSSN_PREC_F64)
+    return mad ? launch_stage_t<double, true>(C, us, O, st, parts) : launch_stage_t<double, false>(C, us, O, st, parts);
+  if (precision == BSSN_PREC_F32RHS)
+    return mad ? launch_stage_t<float, true>(C, us, O, st, parts) : launch_stage_t<float, false>(C, us, O, st, parts);
+  return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
+}
+
+// Points updated by the tile kernel (part 1) or the shell kernel (part 2).
+int64_t stage_points(const Ctx &C, int part) {
+  int64_t shell = 0;
+  if (C.any_shell) {
+    // inclusion-exclusion over the per-axis shells: count the complement
+    int64_t inner = 1;
+    for (int a = 0; a < 3; ++a) inner *= (int64_t)(C.ax[a].n - C.ax[a].shell_lo - C.ax[a].shell_hi);
+    shell = C.ncell - inner;
+  }
+  return part == 2 ? shell : C.ncell - shell;
+}
+
+int launch_project(int64_t n, const double *in, double *out, hipStream_t st) {
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_project, dim3((unsigned)blocks), dim3(256), 0, st, n, in, out);
+  return check_hip(hipGetLastError(), "projection kernel launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *bssn_last_error_string(void) { return g_error.c_str(); }
+
+const char *bssn_version(void) { return "bssn_amd 0.1.0 gfx950"; }
+
+int bssn_rhs(const bssn_params *p, const bssn_grid *grid, int precision, const double *state, double *rhs,
+             void *stream) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (!state || !rhs) return fail(BSSN_ERR_INVALID, "null state or rhs pointer");
+  if (state == rhs) return fail(BSSN_ERR_INVALID, "state and rhs must not alias");
+  StageOut O{nullptr, nullptr, nullptr, rhs, 0, 0};
+  return launch_stage(C, precision, state, O, (hipStream_t)stream);
+}
+
+int bssn_enforce_constraints(const bssn_grid *grid, const double *in, double *out, void *stream) {
+  if (!grid || !in || !out) return fail(BSSN_ERR_INVALID, "null argument");
+  const int64_t n = (int64_t)grid->nx * grid->ny * grid->nz;
+  if (n <= 0) return fail(BSSN_ERR_INVALID, "empty grid");
+  return launch_project(n, in, out, (hipStream_t)stream);
+}
+
+int bssn_rk4_stage(const bssn_params *p, const bssn_grid *grid, int precision, int stage, const double *u_s,
+                   const double *u0, double *ustar, double *u_next, int reproject, void *stream) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (stage < 1 || stage > 4) return fail(BSSN_ERR_INVALID, "stage must be 1..4");
+  if (!u_s || !ustar || !u_next || (stage != 4 && !u0)) return fail(BSSN_ERR_INVALID, "null buffer");
+  if (u_next == u_s) return fail(BSSN_ERR_INVALID, "u_next must not alias u_s");
+  if (stage != 4 && u_next == u0) return fail(BSSN_ERR_INVALID, "u_next must not alias u0 before stage 4");
+  StageOut O{stage == 1 ? u_s : u0, ustar, u_next, nullptr, stage, reproject};
+  return launch_stage(C, precision, u_s, O, (hipStream_t)stream);
+}
+
+int bssn_evolve(const bssn_params *p, const bssn_grid *grid, int precision, int nsteps, double *state,
+                double *work, void *stream) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(BSSN_ERR_INVALID, "nsteps must be non-negative");
+  if (!state || !work) return fail(BSSN_ERR_INVALID, "null buffer");
+  if (nsteps == 0) return BSSN_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = C.ncell * NCOMP;
+  double *ustar = work, *ua = work + n, *ub = work + 2 * n;
+  rc = launch_project(C.ncell, state, state, st);
+  if (rc) return rc;
+  for (int s = 0; s < nsteps; ++s) {
+    const int last = s == nsteps - 1;
+    StageOut O1{state, ustar, ua, nullptr, 1, 0};
+    if ((rc = launch_stage(C, precision, state, O1, st))) return rc;
+    StageOut O2{state, ustar, ub, nullptr, 2, 0};
+    if ((rc = launch_stage(C, precision, ua, O2, st))) return rc;
+    StageOut O3{state, ustar, ua, nullptr, 3, 0};
+    if ((rc = launch_stage(C, precision, ub, O3, st))) return rc;
+    StageOut O4{nullptr, ustar, state, nullptr, 4, last ? 0 : 1};
+    if ((rc = launch_stage(C, precision, ua, O4, st))) return rc;
+  }
+  return BSSN_OK;
+}
+
+int bssn_constraints(const bssn_params *p, const bssn_grid *grid, const double *state, double *out, void *stream) {
+  Ctx C;
+  bssn_params q = p ? *p : bssn_params{};
+  q.kappa = 0.0;  // constraints do not depend on kappa
+  int rc = resolve(&q, grid, &C);
+  if (rc) return rc;
+  if (!state || !out) return fail(BSSN_ERR_INVALID, "null buffer");
+  const int64_t blocks = (C.ncell + 255) / 256;
+  hipLaunchKernelGGL(k_constraints, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, C, state, out);
+  return check_hip(hipGetLastError(), "constraint kernel launch");
+}
+
+int64_t bssn_norm_partial_size(const bssn_grid *grid) {
+  if (!grid) return fail(BSSN_ERR_INVALID, "null grid");
+  return (int64_t)RED_BLOCKS_MAX * 10;
+}
+
+int bssn_constraint_norms(const bssn_grid *grid, const double *constraints, double *partial, double *norms,
+                          void *stream) {
+  if (!grid || !constraints || !partial || !norms) return fail(BSSN_ERR_INVALID, "null argument");
+  const int64_t n = (int64_t)grid->nx * grid->ny * grid->nz;
+  if (n <= 0) return fail(BSSN_ERR_INVALID, "empty grid");
+  int64_t blocks = (n + RED_THREADS - 1) / RED_THREADS;
+  if (blocks > RED_BLOCKS_MAX) blocks = RED_BLOCKS_MAX;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_norm_partial, dim3((unsigned)blocks), dim3(RED_THREADS), 0, st, n, constraints, partial);
+  int rc = check_hip(hipGetLastError(), "norm kernel launch");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, st, (int)blocks, n, partial, norms);
+  return check_hip(hipGetLastError(), "norm kernel launch");
+}
+
+int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precision, int stage, int part,
+                         const double *u_s, const double *u0, double *ustar, double *u_next, int reps, float *ms,
+                         void *stream) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (stage < 1 || stage > 4) return fail(BSSN_ERR_INVALID, "stage must be 1..4");
+  if (part != 1 && part != 2) return fail(BSSN_ERR_INVALID, "part must be 1 (tile) or 2 (shell)");
+  if (reps < 1 || !ms) return fail(BSSN_ERR_INVALID, "reps must be positive and ms non-null");
+  if (!u_s || !ustar || !u_next || (stage != 4 && !u0)) return fail(BSSN_ERR_INVALID, "null buffer");
+  if (u_next == u_s) return fail(BSSN_ERR_INVALID, "u_next must not alias u_s");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  if ((rc = check_hip(hipEventCreate(&e0), "hipEventCreate"))) return rc;
+  if ((rc = check_hip(hipEventCreate(&e1), "hipEventCreate"))) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  StageOut O{stage == 1 ? u_s : u0, ustar, u_next, nullptr, stage, 0};
+  rc = check_hip(hipEventRecord(e0, st), "hipEventRecord");
+  for (int r = 0; r < reps && !rc; ++r) rc = launch_stage(C, precision, u_s, O, st, part);
+  if (!rc) rc = check_hip(hipEventRecord(e1, st), "hipEventRecord");
+  if (!rc) rc = check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+  float t = 0.0f;
+  if (!rc) rc = check_hip(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+  if (!rc) *ms = t / (float)reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int64_t bssn_stage_points(const bssn_params *p, const bssn_grid *grid, int part) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (part != 1 && part != 2) return fail(BSSN_ERR_INVALID, "part must be 1 (tile) or 2 (shell)");
+  return stage_points(C, part);
+}
+
+}  // extern "C"

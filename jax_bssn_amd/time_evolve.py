@@ -1,0 +1,129 @@
+"""Drop-in RHS / projection / RK4 entry points (reference evolution/time_evolve.py).
+
+Every function takes and returns ``BSSNVariables`` in the reference layout
+(NumPy arrays or torch tensors; the result has the kind of the input) and
+runs on the GPU through the C ABI of include/bssn_amd.h.  Inputs are never
+modified.  ``precision="f32rhs"`` selects the FP32-derivative / FP64-state
+variant.  For long runs without host round trips use ``Evolver``, which keeps
+the packed state and the RK4 work buffers resident in HBM.
+"""
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .variables import NCOMP, BSSNParameters, BSSNVariables, device, pack, spatial_shape, unpack
+
+_PRECISIONS = {"f64": _lib.PREC_F64, "f32rhs": _lib.PREC_F32RHS}
+
+
+def precision_code(precision):
+    try:
+        return _PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}") from None
+
+
+def stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _like(v):
+    return v.conformal_factor
+
+
+def rhs_packed(state, params, precision="f64", out=None):
+    """RHS of a packed (21, nx, ny, nz) device state."""
+    lib = _lib.load()
+    shape = tuple(state.shape[1:])
+    if out is None:
+        out = torch.empty_like(state)
+    _lib.check(lib.bssn_rhs(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(shape)),
+                            precision_code(precision), _ptr(state), _ptr(out), stream_handle()))
+    return out
+
+
+def project_packed(state, out=None):
+    lib = _lib.load()
+    if out is None:
+        out = torch.empty_like(state)
+    _lib.check(lib.bssn_enforce_constraints(ctypes.byref(_lib.c_grid(state.shape[1:])), _ptr(state),
+                                            _ptr(out), stream_handle()))
+    return out
+
+
+def evolve_packed(state, params, nsteps, work=None, precision="f64"):
+    """rk4_step applied nsteps times to a packed device state, in place."""
+    lib = _lib.load()
+    if work is None:
+        work = torch.empty((3,) + tuple(state.shape), dtype=torch.float64, device=state.device)
+    _lib.check(lib.bssn_evolve(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(state.shape[1:])),
+                               precision_code(precision), int(nsteps), _ptr(state), _ptr(work),
+                               stream_handle()))
+    return state
+
+
+def compute_bssn_rhs(vars, params, precision="f64"):
+    """Reference time_evolve.py:41-57 (vacuum RHS + Sommerfeld faces)."""
+    state = pack(vars)
+    return unpack(rhs_packed(state, params, precision), like=_like(vars))
+
+
+def enforce_algebraic_constraints(vars):
+    """Reference time_evolve.py:134-141 (det = 1, then trace-free A)."""
+    state = pack(vars)
+    return unpack(project_packed(state, state), like=_like(vars))
+
+
+def rk4_step(vars, params, precision="f64"):
+    """Reference time_evolve.py:144-217."""
+    state = pack(vars)
+    evolve_packed(state, params, 1, precision=precision)
+    return unpack(state, like=_like(vars))
+
+
+def evolve(vars, params, nsteps, precision="f64"):
+    """``nsteps`` applications of rk4_step, with the state resident in HBM."""
+    state = pack(vars)
+    evolve_packed(state, params, nsteps, precision=precision)
+    return unpack(state, like=_like(vars))
+
+
+class Evolver:
+    """Packed device state plus RK4 work buffers for repeated stepping.
+
+    ``stage(s)`` exposes the single fused RK4 stage (bssn_rk4_stage).
+    """
+
+    def __init__(self, vars_or_state, params, precision="f64"):
+        self.params = params
+        self.precision = precision
+        if isinstance(vars_or_state, torch.Tensor):
+            self.state = vars_or_state
+        else:
+            self.state = pack(vars_or_state)
+        self.shape = tuple(self.state.shape[1:])
+        self.work = torch.empty((3, NCOMP) + self.shape, dtype=torch.float64, device=self.state.device)
+        self._cp = _lib.c_params(params)
+        self._cg = _lib.c_grid(self.shape)
+        self._prec = precision_code(precision)
+
+    def step(self, nsteps=1):
+        lib = _lib.load()
+        _lib.check(lib.bssn_evolve(ctypes.byref(self._cp), ctypes.byref(self._cg), self._prec, int(nsteps),
+                                   _ptr(self.state), _ptr(self.work), stream_handle()))
+        return self
+
+    def stage(self, s, u_s, u0, ustar, u_next, reproject=False):
+        lib = _lib.load()
+        _lib.check(lib.bssn_rk4_stage(ctypes.byref(self._cp), ctypes.byref(self._cg), self._prec, int(s),
+                                      _ptr(u_s), _ptr(u0) if u0 is not None else None, _ptr(ustar),
+                                      _ptr(u_next), 1 if reproject else 0, stream_handle()))
+
+    def variables(self, like=None):
+        return unpack(self.state, like=like)

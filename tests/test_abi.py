@@ -1,0 +1,92 @@
+"""CPU-side checks of the C ABI: the in-tree library loads, exports exactly the
+functions include/bssn_amd.h declares, the ctypes mirrors have the C layout,
+and invalid configurations fail before any GPU work (ValueError, as the
+reference's validators raise)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from jax_bssn_amd import _lib
+from jax_bssn_amd.build import INCLUDE, LIB_PATH
+
+HEADER = os.path.join(INCLUDE, "bssn_amd.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bssn_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("library not built (run python -m jax_bssn_amd.build)")
+    return _lib.load()
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(_lib.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (bssn_[a-z0-9_]+)$", out, flags=re.M))
+    for name in _declared():
+        assert name in exported, name
+        assert hasattr(lib, name)
+
+
+def test_struct_layout_matches_header():
+    # doubles first (10 x 8), then 2 + 6 + 1 + 3 + 6 + 3 int32
+    assert ctypes.sizeof(_lib.CParams) == 10 * 8 + 21 * 4 + 4  # padded to 8
+    assert _lib.CParams.bc.offset == 88
+    assert ctypes.sizeof(_lib.CGrid) == 12
+
+
+def test_version_string(lib):
+    assert b"gfx950" in lib.bssn_version()
+
+
+def _call_rhs(lib, params, shape=(16, 16, 16)):
+    return lib.bssn_rhs(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(shape)), 0,
+                        None, None, None)
+
+
+def test_invalid_configurations_raise_value_error(lib):
+    from jax_bssn_amd.variables import BSSNParameters
+    bad = [
+        (BSSNParameters(dx=-1.0), "dx"),
+        (BSSNParameters(dx=0.1, xl_bc=2), "boundary codes"),
+        (BSSNParameters(dx=0.1, gauge=3), "gauge"),
+        (BSSNParameters(dx=0.1, reduced_axes=(True, False, False), xl_bc=1), "reduced"),
+        (BSSNParameters(dx=0.1, guard_cells=4, owned_shape=(16, 16, 16)), "owned range"),
+    ]
+    for params, word in bad:
+        with pytest.raises(ValueError, match=word):
+            _lib.check(_call_rhs(lib, params))
+    with pytest.raises(ValueError, match="at least 8"):
+        _lib.check(_call_rhs(lib, BSSNParameters(dx=0.1, zl_bc=1), shape=(16, 16, 6)))
+    # a valid configuration reaches the pointer checks, still before any launch
+    with pytest.raises(ValueError, match="null"):
+        _lib.check(_call_rhs(lib, BSSNParameters(dx=0.1)))
+
+
+def test_stage_argument_validation(lib):
+    from jax_bssn_amd.variables import BSSNParameters
+    p = ctypes.byref(_lib.c_params(BSSNParameters(dx=0.1)))
+    g = ctypes.byref(_lib.c_grid((8, 8, 8)))
+    with pytest.raises(ValueError, match="stage"):
+        _lib.check(lib.bssn_rk4_stage(p, g, 0, 5, None, None, None, None, 0, None))
+    buf = ctypes.c_void_p(0x1000)
+    with pytest.raises(ValueError, match="alias"):
+        _lib.check(lib.bssn_rk4_stage(p, g, 0, 2, buf, ctypes.c_void_p(0x2000), ctypes.c_void_p(0x3000),
+                                      buf, 0, None))
+    with pytest.raises(ValueError, match="precision"):
+        _lib.check(lib.bssn_rk4_stage(p, g, 7, 1, buf, buf, ctypes.c_void_p(0x3000),
+                                      ctypes.c_void_p(0x4000), 0, None) if False else
+                   lib.bssn_rhs(p, g, 7, buf, ctypes.c_void_p(0x4000), None))
