@@ -19,110 +19,12 @@
 #include "bssn_amd.h"
 #include "bssn_common.h"
 #include "rhs_point.h"
+#include "stage_common.h"
 #include "stencil_providers.h"
 
 namespace bssn {
 
-// ------------------------------------------------------------ projection
-
-// Reference time_evolve.py:94-131: gamma <- gamma det^(-1/3); then
-// A <- A - 1/3 gamma (gamma^ij A_ij) with the inverse of the rescaled metric.
-__device__ __forceinline__ void project(double (&g)[6], double (&A)[6]) {
-  const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
-                     g[2] * (g[1] * g[4] - g[3] * g[2]);
-  const double s = pow(det, -1.0 / 3.0);
-#pragma unroll
-  for (int c = 0; c < 6; ++c) g[c] *= s;
-  double ig[6];
-  inverse_metric(g, ig);
-  const double tr = contract(ig, A);
-#pragma unroll
-  for (int c = 0; c < 6; ++c) A[c] -= (1.0 / 3.0) * g[c] * tr;
-}
-
-// ------------------------------------------------------------ stage update
-
-// Receives each component's RHS as soon as it is final and applies the
-// stage update of bssn_rk4_stage (include/bssn_amd.h) right away; the
-// metric and A are held until done() so they can be projected together.
-struct Emitter {
-  int64_t n, idx;
-  double dt;
-  const double *u0;
-  double *ustar, *unext, *rhs;
-  int mode, reproject;
-  bool active;
-  double gn[6], An[6];
-
-  __device__ Emitter(const Ctx &C, const StageOut &O, int64_t i, bool act)
-      : n(C.ncell), idx(i), dt(C.dt), u0(O.u0), ustar(O.ustar), unext(O.unext), rhs(O.rhs), mode(O.mode),
-        reproject(O.reproject), active(act) {}
-
-  __device__ void put(int c, double k) {
-    if (!active) return;
-    const int64_t at = c * n + idx;
-    if (mode == 0) {
-      rhs[at] = k;
-      return;
-    }
-    double un;
-    if (mode == 1) {
-      const double v0 = u0[at];
-      ustar[at] = v0 + (dt / 6.0) * k;
-      un = v0 + (0.5 * dt) * k;
-    } else if (mode == 4) {
-      un = ustar[at] + (dt / 6.0) * k;
-    } else {
-      const double v0 = u0[at];
-      ustar[at] = ustar[at] + (dt / 3.0) * k;
-      un = v0 + (mode == 2 ? 0.5 * dt : dt) * k;
-    }
-    if (c <= GZZ) {
-      gn[c] = un;
-    } else if (c >= AXX && c <= AZZ) {
-      An[c - AXX] = un;
-    } else {
-      unext[at] = un;
-    }
-  }
-
-  __device__ void done() {
-    if (!active || mode == 0) return;
-    project(gn, An);
-    if (mode == 4 && reproject) project(gn, An);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      unext[(GXX + c) * n + idx] = gn[c];
-      unext[(AXX + c) * n + idx] = An[c];
-    }
-  }
-};
-
 // ----------------------------------------------------------- shell logic
-
-__device__ __forceinline__ bool in_shell(const Ctx &C, int x, int y, int z) {
-  const int p[3] = {x, y, z};
-  bool s = false;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const Axis &A = C.ax[a];
-    const int lo = p[a] - A.start, hi = A.end - 1 - p[a];
-    s |= (lo >= 0 && lo < A.shell_lo) || (hi >= 0 && hi < A.shell_hi);
-  }
-  return s;
-}
-
-__device__ __forceinline__ bool sommerfeld_point(const Ctx &C, int x, int y, int z) {
-  const int p[3] = {x, y, z};
-  bool face = false, owned = true;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const Axis &A = C.ax[a];
-    face |= (A.somm_lo && p[a] == A.start) || (A.somm_hi && p[a] == A.end - 1);
-    owned &= (p[a] >= A.start && p[a] < A.end);
-  }
-  return face && owned;
-}
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of tiles so tiles sharing halos share an L2.
@@ -174,31 +76,9 @@ __global__ void __launch_bounds__(TX *TY *TZ)
 
 // ----------------------------------------------------------- shell kernel
 
-struct Slabs {
-  int count;
-  int lo[6][3], hi[6][3];
-  int64_t first[7];  // prefix sums of slab sizes
-};
-
 __device__ __forceinline__ bool in_slab(const Slabs &S, int s, int x, int y, int z) {
   return x >= S.lo[s][0] && x < S.hi[s][0] && y >= S.lo[s][1] && y < S.hi[s][1] && z >= S.lo[s][2] &&
          z < S.hi[s][2];
-}
-
-// Sommerfeld RHS of component c (reference evolution/boundaries.py:29-92).
-template <typename T>
-__device__ T sommerfeld_rhs(PointProvider<T> &pv, const Ctx &C, int c, double uc, int x, int y, int z) {
-  pv.stage_comp(c);
-  const T d0 = pv.d1(0), d1 = pv.d1(1), d2 = pv.d1(2);
-  const T X = (T)(C.xmin[0] + C.dx * (double)x);
-  const T Y = (T)(C.xmin[1] + C.dx * (double)y);
-  const T Z = (T)(C.xmin[2] + C.dx * (double)z);
-  const T r = sqrt(X * X + Y * Y + Z * Z);
-  const T rs = r > T(0) ? r : T(1);
-  const bool unit = (c == GXX || c == GYY || c == GZZ || c == CW || c == ALPHA);
-  const T u0 = unit ? T(1) : T(0);
-  const T radial = (X * d0 + Y * d1 + Z * d2) / rs;
-  return -(radial + ((T)uc - u0) / rs);
 }
 
 template <typename T>
@@ -468,106 +348,16 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
+  char err[512] = {0};
+  const int rc = bssn::resolve(p, grid, C, err, sizeof err);
+  if (rc) g_error = err;
+  return rc;
+}
+
 int check_hip(hipError_t e, const char *what) {
   if (e != hipSuccess) return fail(BSSN_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
   return BSSN_OK;
-}
-
-// Reference evolution/spatial_derivatives.py:18-41 (_boundary_data),
-// evolution/boundaries.py:44-72 (mask) and the validators of
-// parallel/initialization.py:31-37 / parallel/layout.py:92-108.
-int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
-  if (!p || !grid || !C) return fail(BSSN_ERR_INVALID, "null argument");
-  const int n[3] = {grid->nx, grid->ny, grid->nz};
-  for (int d = 0; d < 3; ++d)
-    if (n[d] < 1) return fail(BSSN_ERR_INVALID, "grid extent %d along axis %d must be positive", n[d], d);
-  const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
-  if (ncell >= (int64_t(1) << 31)) return fail(BSSN_ERR_INVALID, "grid of %lld points exceeds 2^31", (long long)ncell);
-  if (!(p->dx > 0.0) || !std::isfinite(p->dx)) return fail(BSSN_ERR_INVALID, "dx must be finite and positive");
-  if (p->guard_cells < 0) return fail(BSSN_ERR_INVALID, "guard_cells must be non-negative");
-  std::memset(C, 0, sizeof(Ctx));
-  const bool owned_given = p->owned_shape[0] >= 0;
-  int any_shell = 0;
-  for (int d = 0; d < 3; ++d) {
-    Axis &A = C->ax[d];
-    const int lc = p->bc[2 * d], rc = p->bc[2 * d + 1];
-    if ((lc != 0 && lc != 1) || (rc != 0 && rc != 1))
-      return fail(BSSN_ERR_INVALID, "boundary codes must be 0 (periodic) or 1 (Sommerfeld)");
-    const int count = owned_given ? p->owned_shape[d] : n[d];
-    A.n = n[d];
-    A.start = p->guard_cells;
-    A.end = A.start + count;
-    if (count < 1 || A.end > n[d])
-      return fail(BSSN_ERR_INVALID, "owned range [%d, %d) does not fit axis %d of extent %d", A.start, A.end, d, n[d]);
-    const bool global_somm = lc == 1 || rc == 1;
-    A.q = global_somm ? 1.0 : p->mad_q;
-    A.lbc = (lc == 1 && p->owned_faces[2 * d]) ? 1 : 0;
-    A.rbc = (rc == 1 && p->owned_faces[2 * d + 1]) ? 1 : 0;
-    if (p->reduced_axes[0] >= 0)
-      A.reduced = p->reduced_axes[d] ? 1 : 0;
-    else
-      A.reduced = (owned_given && count == 1) ? 1 : 0;
-    A.somm_lo = (A.lbc && (!owned_given || count > 1)) ? 1 : 0;
-    A.somm_hi = (A.rbc && (!owned_given || count > 1)) ? 1 : 0;
-    if (A.reduced && global_somm)
-      return fail(BSSN_ERR_INVALID, "axis %d is reduced and cannot carry a Sommerfeld face", d);
-    if ((A.lbc || A.rbc) && count < 8)
-      return fail(BSSN_ERR_INVALID, "Sommerfeld closures on axis %d need at least 8 owned points (have %d)", d, count);
-    A.shell_lo = A.lbc ? 3 : 0;
-    A.shell_hi = A.rbc ? 3 : 0;
-    any_shell |= A.lbc | A.rbc;
-  }
-  C->ax[0].stride = n[1] * n[2];
-  C->ax[1].stride = n[2];
-  C->ax[2].stride = 1;
-  C->ncell = ncell;
-  C->dx = p->dx;
-  C->inv_dx = 1.0 / p->dx;
-  C->inv_dx2 = 1.0 / (p->dx * p->dx);
-  C->ko = p->nu / 64.0 * std::pow(p->dx, 5) / std::pow(p->dx, 6);
-  C->eta = p->eta;
-  C->g = p->g;
-  C->kappa = p->kappa;
-  C->dt = p->dt;
-  C->xmin[0] = p->x_min;
-  C->xmin[1] = p->y_min;
-  C->xmin[2] = p->z_min;
-  C->gauge = p->gauge;
-  C->zero_shift = p->zero_shift;
-  C->any_shell = any_shell;
-  if (p->gauge != 0 && p->gauge != 1) return fail(BSSN_ERR_INVALID, "gauge must be 0 (harmonic) or 1 (1+log)");
-  if (p->kappa != 0.0)
-    return fail(BSSN_ERR_UNSUPPORTED, "kappa != 0 (constraint damping) is not implemented yet");
-  return BSSN_OK;
-}
-
-Slabs make_slabs(const Ctx &C) {
-  Slabs S;
-  std::memset(&S, 0, sizeof S);
-  int64_t total = 0;
-  for (int a = 0; a < 3; ++a) {
-    for (int side = 0; side < 2; ++side) {
-      const Axis &A = C.ax[a];
-      const int w = side == 0 ? A.shell_lo : A.shell_hi;
-      if (!w) continue;
-      const int s = S.count++;
-      for (int b = 0; b < 3; ++b) {
-        S.lo[s][b] = 0;
-        S.hi[s][b] = C.ax[b].n;
-      }
-      if (side == 0) {
-        S.lo[s][a] = A.start;
-        S.hi[s][a] = A.start + w;
-      } else {
-        S.lo[s][a] = A.end - w;
-        S.hi[s][a] = A.end;
-      }
-      S.first[s] = total;
-      total += (int64_t)(S.hi[s][0] - S.lo[s][0]) * (S.hi[s][1] - S.lo[s][1]) * (S.hi[s][2] - S.lo[s][2]);
-    }
-  }
-  S.first[S.count] = total;
-  return S;
 }
 
 constexpr int TX = 4, TY = 8, TZ = 8;

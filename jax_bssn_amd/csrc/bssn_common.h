@@ -6,6 +6,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Functions shared by the kernels and the host build of the per-point code
+// (used by the tests to check the algebra on the CPU).
+#define BSSN_HD __host__ __device__
+
 namespace bssn {
 
 // Packed-state component indices (the reference field order,

@@ -24,7 +24,7 @@ namespace bssn {
 
 // Reference tensor_algebra.py:20-71 (explicit Cholesky, upper triangle).
 template <typename T>
-__device__ __forceinline__ void inverse_metric(const T (&g)[6], T (&ig)[6]) {
+BSSN_HD __forceinline__ void inverse_metric(const T (&g)[6], T (&ig)[6]) {
   T L00 = sqrt(g[0]);
   T L10 = g[1] / L00;
   T L20 = g[2] / L00;
@@ -47,13 +47,13 @@ __device__ __forceinline__ void inverse_metric(const T (&g)[6], T (&ig)[6]) {
 
 // Full contraction X^{ij} Y_ij of two symmetric tensors.
 template <typename T>
-__device__ __forceinline__ T contract(const T (&X)[6], const T (&Y)[6]) {
+BSSN_HD __forceinline__ T contract(const T (&X)[6], const T (&Y)[6]) {
   return X[0] * Y[0] + X[3] * Y[3] + X[5] * Y[5] + T(2) * (X[1] * Y[1] + X[2] * Y[2] + X[4] * Y[4]);
 }
 
 // v_m X^{mn} w_n for a symmetric X.
 template <typename T>
-__device__ __forceinline__ T contract_vec(const T (&X)[6], const T (&v)[3], const T (&w)[3]) {
+BSSN_HD __forceinline__ T contract_vec(const T (&X)[6], const T (&v)[3], const T (&w)[3]) {
   T acc = T(0);
 #pragma unroll
   for (int m = 0; m < 3; ++m)
@@ -67,7 +67,7 @@ constexpr int SYM_J[6] = {0, 1, 2, 1, 2, 2};
 
 // Second derivatives of the staged component: s[sym(m, n)].
 template <typename T, class P>
-__device__ __forceinline__ void second_derivatives(const P &pv, T (&s)[6]) {
+BSSN_HD __forceinline__ void second_derivatives(const P &pv, T (&s)[6]) {
   s[0] = pv.d2(0);
   s[3] = pv.d2(1);
   s[5] = pv.d2(2);
@@ -77,7 +77,7 @@ __device__ __forceinline__ void second_derivatives(const P &pv, T (&s)[6]) {
 }
 
 template <typename T, class P>
-__device__ __forceinline__ T advection(const P &pv, const T (&b)[3]) {
+BSSN_HD __forceinline__ T advection(const P &pv, const T (&b)[3]) {
   return b[0] * pv.upw(0, b[0]) + b[1] * pv.upw(1, b[1]) + b[2] * pv.upw(2, b[2]);
 }
 
@@ -86,7 +86,7 @@ __device__ __forceinline__ T advection(const P &pv, const T (&b)[3]) {
 // as component c is final (so its stage update leaves the registers early)
 // and out.done() after the last one.  u[] are the point values.
 template <typename T, class P, class E>
-__device__ __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&u)[NCOMP], E &out) {
+BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&u)[NCOMP], E &out) {
   T g[6], A[6], G[3], b[3];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {

@@ -214,7 +214,7 @@ struct Row {
   int n;
   int off[8];
   double c[8];
-  __device__ void push(int o, double v) {
+  BSSN_HD void push(int o, double v) {
     for (int k = 0; k < n; ++k)
       if (off[k] == o) { c[k] += v; return; }
     off[n] = o;
@@ -223,12 +223,12 @@ struct Row {
   }
 };
 
-__device__ inline void row_add(Row &r, const int *off, const double *c, int m, double scale) {
+BSSN_HD inline void row_add(Row &r, const int *off, const double *c, int m, double scale) {
   for (int k = 0; k < m; ++k) r.push(off[k], c[k] * scale);
 }
 
 // Centred D1 row at position p of axis A (closures, MAD).
-__device__ inline Row row_d1(const Axis &A, int p, double inv_dx, bool force_d4) {
+BSSN_HD inline Row row_d1(const Axis &A, int p, double inv_dx, bool force_d4) {
   Row r;
   r.n = 0;
   const int s = A.start, e = A.end;
@@ -263,7 +263,7 @@ __device__ inline Row row_d1(const Axis &A, int p, double inv_dx, bool force_d4)
   return r;
 }
 
-__device__ inline Row row_d2(const Axis &A, int p, double inv_dx2) {
+BSSN_HD inline Row row_d2(const Axis &A, int p, double inv_dx2) {
   Row r;
   r.n = 0;
   const int s = A.start, e = A.end;
@@ -298,7 +298,7 @@ __device__ inline Row row_d2(const Axis &A, int p, double inv_dx2) {
   return r;
 }
 
-__device__ inline Row row_upwind(const Axis &A, int p, double inv_dx, double coeff) {
+BSSN_HD inline Row row_upwind(const Axis &A, int p, double inv_dx, double coeff) {
   const int s = A.start, e = A.end;
   const int w = A.n < 3 ? A.n : 3;
   if ((A.lbc && p >= s && p < s + w) || (A.rbc && p >= e - w && p < e))
@@ -335,7 +335,7 @@ __device__ inline Row row_upwind(const Axis &A, int p, double inv_dx, double coe
 }
 
 // Sixth difference row scaled by dx^6 (i.e. the bare integer stencil).
-__device__ inline Row row_d6(const Axis &A, int p) {
+BSSN_HD inline Row row_d6(const Axis &A, int p) {
   const double L[3][8] = {{4.0, -27.0, 78.0, -125.0, 120.0, -69.0, 22.0, -3.0},
                           {3.0, -20.0, 57.0, -90.0, 85.0, -48.0, 15.0, -2.0},
                           {2.0, -13.0, 36.0, -55.0, 50.0, -27.0, 8.0, -1.0}};
@@ -344,7 +344,7 @@ __device__ inline Row row_d6(const Axis &A, int p) {
   const int s = A.start, e = A.end;
   for (int k = 0; k < 3; ++k) {
     if (A.rbc && p == e - 1 - k) {
-      for (int t = 0; t < 8; ++t) r.push(-t, L[k][t]);
+      for (int t = 0; t < 8; ++t) r.push(k - t, L[k][t]);  // taps e-1-t of the point e-1-k
       return r;
     }
   }
@@ -368,42 +368,42 @@ struct PointProvider {
   int p[3];
   int64_t idx;
 
-  __device__ void setup(const Ctx &ctx, const double *u, int x, int y, int z) {
+  BSSN_HD void setup(const Ctx &ctx, const double *u, int x, int y, int z) {
     C = &ctx;
     src = u;
     f = u;
     p[0] = x; p[1] = y; p[2] = z;
     idx = (int64_t)x * ctx.ax[0].stride + (int64_t)y * ctx.ax[1].stride + z;
   }
-  __device__ void stage(int v) { f = src + visit_comp(v) * C->ncell; }
-  __device__ void stage_comp(int comp) { f = src + comp * C->ncell; }
+  BSSN_HD void stage(int v) { f = src + visit_comp(v) * C->ncell; }
+  BSSN_HD void stage_comp(int comp) { f = src + comp * C->ncell; }
 
-  __device__ T ld(int a, int oa) const {
+  BSSN_HD T ld(int a, int oa) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
     return (T)f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]];
   }
-  __device__ T ld2(int a, int oa, int b, int ob) const {
+  BSSN_HD T ld2(int a, int oa, int b, int ob) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
     q[b] = wrap_index(q[b] + ob, C->ax[b].n);
     return (T)f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]];
   }
-  __device__ T apply(int a, const Row &r) const {
+  BSSN_HD T apply(int a, const Row &r) const {
     T s = T(0);
     for (int k = 0; k < r.n; ++k) s += (T)r.c[k] * ld(a, r.off[k]);
     return s;
   }
-  __device__ T val() const { return (T)f[idx]; }
-  __device__ T d1(int a) const {
+  BSSN_HD T val() const { return (T)f[idx]; }
+  BSSN_HD T d1(int a) const {
     if (C->ax[a].reduced) return T(0);
     return apply(a, row_d1(C->ax[a], p[a], C->inv_dx, false));
   }
-  __device__ T d2(int a) const {
+  BSSN_HD T d2(int a) const {
     if (C->ax[a].reduced) return T(0);
     return apply(a, row_d2(C->ax[a], p[a], C->inv_dx2));
   }
-  __device__ T dmix(int a, int b) const {
+  BSSN_HD T dmix(int a, int b) const {
     if (C->ax[a].reduced || C->ax[b].reduced) return T(0);
     Row ra = row_d1(C->ax[a], p[a], C->inv_dx, false);
     Row rb = row_d1(C->ax[b], p[b], C->inv_dx, false);
@@ -415,11 +415,11 @@ struct PointProvider {
     }
     return s;
   }
-  __device__ T upw(int a, T coeff) const {
+  BSSN_HD T upw(int a, T coeff) const {
     if (C->ax[a].reduced) return T(0);
     return apply(a, row_upwind(C->ax[a], p[a], C->inv_dx, (double)coeff));
   }
-  __device__ T ko() const {
+  BSSN_HD T ko() const {
     T s = T(0);
     for (int a = 0; a < 3; ++a) {
       if (C->ax[a].reduced) continue;
