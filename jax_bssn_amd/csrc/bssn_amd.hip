@@ -22,6 +22,11 @@
 #include "stage_common.h"
 #include "stencil_providers.h"
 
+// Minimum waves per SIMD the tile kernels are compiled for (caps VGPRs).
+#ifndef BSSN_MIN_WAVES
+#define BSSN_MIN_WAVES 2
+#endif
+
 namespace bssn {
 
 // ----------------------------------------------------------- shell logic
@@ -37,26 +42,49 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
 
 __device__ __forceinline__ void load_point(const Ctx &C, const double *__restrict__ u, int64_t idx,
                                            double (&v)[NCOMP]) {
+  const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)u;
 #pragma unroll
-  for (int c = 0; c < NCOMP; ++c) v[c] = u[c * C.ncell + idx];
+  for (int c = 0; c < NCOMP; ++c) v[c] = g[c * C.ncell + idx];
 }
 
 // ------------------------------------------------------------ tile kernel
 
-template <typename T, int TX, int TY, int TZ, bool MAD>
-__global__ void __launch_bounds__(TX *TY *TZ)
-    k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, int remap) {
-  using P = TileProvider<T, TX, TY, TZ, MAD>;
+// tile_class: 0 every tile; 1 only tiles without closure points; 2 only
+// tiles with closure points (the two classes of one Sommerfeld stage run
+// as a lean interior launch plus a closure-capable face launch).
+template <typename T, int TX, int TY, int TZ, bool MAD, bool CL>
+__global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
+    k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, int remap, int tile_class) {
+  using P = TileProvider<T, TX, TY, TZ, MAD, CL>;
   __shared__ T smem[P::LDS_ELEMS];
+  __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   const int nbx = (C.ax[0].n + TX - 1) / TX;
   const int nby = (C.ax[1].n + TY - 1) / TY;
   const int nbz = (C.ax[2].n + TZ - 1) / TZ;
   const int ntiles = nbx * nby * nbz;
   const int bid = remap ? xcd_remap((int)blockIdx.x, ntiles) : (int)blockIdx.x;
-  const int bz = bid % nbz;
-  const int bt = bid / nbz;
-  const int by = bt % nby;
-  const int bx = bt / nby;
+  int bx, by, bz;
+  if (C.tile_order == 0) {  // z fastest
+    bz = bid % nbz;
+    by = (bid / nbz) % nby;
+    bx = bid / (nbz * nby);
+  } else {  // x fastest
+    bx = bid % nbx;
+    bz = (bid / nbx) % nbz;
+    by = bid / (nbx * nbz);
+  }
+  if (tile_class) {
+    const int o[3] = {bx * TX, by * TY, bz * TZ};
+    const int t[3] = {TX, TY, TZ};
+    bool touch = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const Axis &A = C.ax[a];
+      touch |= (A.lbc && o[a] < A.start + 3 && o[a] + t[a] > A.start) ||
+               (A.rbc && o[a] + t[a] > A.end - 3 && o[a] < A.end);
+    }
+    if (touch != (tile_class == 2)) return;  // uniform per block, before any barrier
+  }
   const int tid = threadIdx.x;
   const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
   const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
@@ -67,10 +95,12 @@ __global__ void __launch_bounds__(TX *TY *TZ)
   const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
 
   P pv;
-  pv.setup(C, us, smem, bx * TX, by * TY, bz * TZ, tx, ty, tz);
+  pv.setup(C, us, smem, stash, bx * TX, by * TY, bz * TZ, tx, ty, tz);
   double u[NCOMP];
   load_point(C, us, idx, u);
-  Emitter out(C, O, idx, inside && !(C.any_shell && in_shell(C, x, y, z)));
+  // without in-tile closures the shell points belong to the shell kernel
+  const bool mine = inside && (CL || !(C.any_shell && in_shell(C, x, y, z)));
+  Emitter<LdsStash<TX * TY * TZ>> out(C, O, idx, mine, pv.face(), pv.st);
   bssn_rhs_point<T>(pv, C, u, out);
 }
 
@@ -99,14 +129,8 @@ __global__ void __launch_bounds__(256) k_stage_shell(Ctx C, const double *__rest
   pv.setup(C, us, x, y, z);
   double u[NCOMP];
   load_point(C, us, pv.idx, u);
-  Emitter out(C, O, pv.idx, true);
-  if (sommerfeld_point(C, x, y, z)) {
-    // the reference replaces the whole RHS on active faces
-    for (int c = 0; c < NCOMP; ++c) out.put(c, (double)sommerfeld_rhs<T>(pv, C, c, u[c], x, y, z));
-    out.done();
-  } else {
-    bssn_rhs_point<T>(pv, C, u, out);
-  }
+  Emitter<RegStash> out(C, O, pv.idx, true, pv.face(), pv.st);
+  bssn_rhs_point<T>(pv, C, u, out);
 }
 
 // ------------------------------------------------------ projection kernel
@@ -248,9 +272,10 @@ __global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__rest
     double first = 0.0;
     for (int j = 0; j < 3; ++j) {
       if (C.ax[j].reduced) continue;
-      Row row = row_d1(C.ax[j], j == 0 ? x : (j == 1 ? y : z), C.inv_dx, false);
+      const Row8<double> row = row_d1<double>(C.ax[j], j == 0 ? x : (j == 1 ? y : z), C.inv_dx, false);
       double s = 0.0;
-      for (int k = 0; k < row.n; ++k) s += row.c[k] * b_at(j, row.off[k], r, j);
+      for (int k = 0; k < 8; ++k)
+        if (row.c[k] != 0.0) s += row.c[k] * b_at(j, row.o0 + k, r, j);
       first += s;
     }
     double Bw = 0.0;  // B_r^j d_j W
@@ -360,19 +385,50 @@ int check_hip(hipError_t e, const char *what) {
   return BSSN_OK;
 }
 
-constexpr int TX = 4, TY = 8, TZ = 8;
+// Tile shape and the minimum waves per SIMD the tile kernels are compiled
+// for (overridable at build time to produce tuning variants).
+#ifndef BSSN_TILE_X
+#define BSSN_TILE_X 4
+#define BSSN_TILE_Y 8
+#define BSSN_TILE_Z 8
+#endif
+constexpr int TX = BSSN_TILE_X, TY = BSSN_TILE_Y, TZ = BSSN_TILE_Z;
 
-template <typename T, bool MAD>
+// Closures can be evaluated inside the tiles when every closure axis has its
+// owned range aligned to the tile grid (each face tile starts / ends on the
+// face); otherwise the closure shells go to the shell kernel.
+bool tiles_aligned(const Ctx &C) {
+  const int t[3] = {TX, TY, TZ};
+  for (int a = 0; a < 3; ++a) {
+    const Axis &A = C.ax[a];
+    if (!(A.lbc || A.rbc)) continue;
+    if (A.start % t[a] != 0 || A.end % t[a] != 0) return false;
+  }
+  return true;
+}
+
+template <typename T, bool MAD, bool CL>
 int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
   const int nbx = (C.ax[0].n + TX - 1) / TX, nby = (C.ax[1].n + TY - 1) / TY, nbz = (C.ax[2].n + TZ - 1) / TZ;
   const int ntiles = nbx * nby * nbz;
   int rc = BSSN_OK;
   if (parts & 1) {
-    hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us, O, 1);
+    if (CL) {
+      // lean kernel on the interior tiles, closure-capable kernel on the face tiles
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us, O,
+                         1, 1);
+      rc = check_hip(hipGetLastError(), "tile kernel launch");
+      if (rc) return rc;
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us, O,
+                         1, 2);
+    } else {
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us,
+                         O, 1, 0);
+    }
     rc = check_hip(hipGetLastError(), "tile kernel launch");
     if (rc) return rc;
   }
-  if ((parts & 2) && C.any_shell) {
+  if ((parts & 2) && C.any_shell && !CL) {
     Slabs S = make_slabs(C);
     const int64_t total = S.first[S.count];
     if (total > 0) {
@@ -384,19 +440,24 @@ int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hipStream_
   return rc;
 }
 
-int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &O, hipStream_t st, int parts = 3) {
+template <typename T>
+int launch_stage_p(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
   const bool mad = C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0;
-  if (precision == BSSN_PREC_F64)
-    return mad ? launch_stage_t<double, true>(C, us, O, st, parts) : launch_stage_t<double, false>(C, us, O, st, parts);
-  if (precision == BSSN_PREC_F32RHS)
-    return mad ? launch_stage_t<float, true>(C, us, O, st, parts) : launch_stage_t<float, false>(C, us, O, st, parts);
+  const bool cl = C.any_shell && tiles_aligned(C);
+  if (mad) return cl ? launch_stage_t<T, true, true>(C, us, O, st, parts) : launch_stage_t<T, true, false>(C, us, O, st, parts);
+  return cl ? launch_stage_t<T, false, true>(C, us, O, st, parts) : launch_stage_t<T, false, false>(C, us, O, st, parts);
+}
+
+int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &O, hipStream_t st, int parts = 3) {
+  if (precision == BSSN_PREC_F64) return launch_stage_p<double>(C, us, O, st, parts);
+  if (precision == BSSN_PREC_F32RHS) return launch_stage_p<float>(C, us, O, st, parts);
   return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
 }
 
 // Points updated by the tile kernel (part 1) or the shell kernel (part 2).
 int64_t stage_points(const Ctx &C, int part) {
   int64_t shell = 0;
-  if (C.any_shell) {
+  if (C.any_shell && !tiles_aligned(C)) {
     // inclusion-exclusion over the per-axis shells: count the complement
     int64_t inner = 1;
     for (int a = 0; a < 3; ++a) inner *= (int64_t)(C.ax[a].n - C.ax[a].shell_lo - C.ax[a].shell_hi);

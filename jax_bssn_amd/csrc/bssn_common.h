@@ -10,6 +10,16 @@
 // (used by the tests to check the algebra on the CPU).
 #define BSSN_HD __host__ __device__
 
+// Address spaces made explicit in device code so that LDS and global
+// accesses compile to ds_* / global_* instructions rather than flat ones.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define BSSN_LDS __attribute__((address_space(3)))
+#define BSSN_GLOBAL __attribute__((address_space(1)))
+#else
+#define BSSN_LDS
+#define BSSN_GLOBAL
+#endif
+
 namespace bssn {
 
 // Packed-state component indices (the reference field order,
@@ -26,9 +36,15 @@ enum Comp : int {
 };
 
 // sym(i, j): index 0..5 of the symmetric pair (i, j) in xx xy xz yy yz zz order.
+// Closed form (no recursion) so that it folds to a constant in unrolled loops.
 __host__ __device__ constexpr int sym(int i, int j) {
-  return (i > j) ? sym(j, i) : (i == 0 ? j : (i == 1 ? 2 + j : 5));
+  return (i < j ? i : j) * (5 - (i < j ? i : j)) / 2 + (i < j ? j : i);
 }
+
+// Stash slots of the per-point algebra: 0..5 metric of the stage update,
+// 6..11 A, 12..14 shift, 15..20 Lie terms of the A RHS, 21 partial K RHS.
+constexpr int STASH_SLOTS = 22;
+enum StashSlot : int { ST_GN = 0, ST_A = 6, ST_B = 12, ST_ALIE = 15, ST_RK = 21 };
 
 // One spatial axis after the reference's _boundary_data resolution
 // (evolution/spatial_derivatives.py:18-41).
@@ -49,9 +65,11 @@ struct Ctx {
   double dx, inv_dx, inv_dx2;
   double ko;        // nu / 64 * dx^5 / dx^6
   double eta, g, kappa, dt;
+  double dt6, dt3, dt2;  // RK4 weights dt/6, dt/3, dt/2
   double xmin[3];
   int gauge, zero_shift;
   int any_shell;    // some axis has a closure shell
+  int tile_order;   // tile traversal: 0 z fastest, 1 x fastest
 };
 
 // Stage-update arguments (see bssn_rk4_stage in include/bssn_amd.h).

@@ -22,27 +22,22 @@
 
 namespace bssn {
 
-// Reference tensor_algebra.py:20-71 (explicit Cholesky, upper triangle).
+// Inverse of the symmetric conformal metric from its upper triangle (the
+// reference factors it by Cholesky, tensor_algebra.py:20-71; the cofactor
+// form used here needs one reciprocal instead of nine divisions and agrees
+// with it to roundoff).
 template <typename T>
 BSSN_HD __forceinline__ void inverse_metric(const T (&g)[6], T (&ig)[6]) {
-  T L00 = sqrt(g[0]);
-  T L10 = g[1] / L00;
-  T L20 = g[2] / L00;
-  T L11 = sqrt(g[3] - L10 * L10);
-  T L21 = (g[4] - L20 * L10) / L11;
-  T L22 = sqrt(g[5] - L20 * L20 - L21 * L21);
-  T M00 = T(1) / L00;
-  T M10 = -L10 / (L00 * L11);
-  T M11 = T(1) / L11;
-  T M20 = (L10 * L21 - L20 * L11) / (L00 * L11 * L22);
-  T M21 = -L21 / (L11 * L22);
-  T M22 = T(1) / L22;
-  ig[0] = M00 * M00 + M10 * M10 + M20 * M20;
-  ig[1] = M10 * M11 + M20 * M21;
-  ig[2] = M20 * M22;
-  ig[3] = M11 * M11 + M21 * M21;
-  ig[4] = M21 * M22;
-  ig[5] = M22 * M22;
+  const T c00 = g[3] * g[5] - g[4] * g[4];
+  const T c01 = g[2] * g[4] - g[1] * g[5];
+  const T c02 = g[1] * g[4] - g[2] * g[3];
+  const T idet = T(1) / (g[0] * c00 + g[1] * c01 + g[2] * c02);
+  ig[0] = c00 * idet;
+  ig[1] = c01 * idet;
+  ig[2] = c02 * idet;
+  ig[3] = (g[0] * g[5] - g[2] * g[2]) * idet;
+  ig[4] = (g[1] * g[2] - g[0] * g[4]) * idet;
+  ig[5] = (g[0] * g[3] - g[1] * g[1]) * idet;
 }
 
 // Full contraction X^{ij} Y_ij of two symmetric tensors.
@@ -82,9 +77,13 @@ BSSN_HD __forceinline__ T advection(const P &pv, const T (&b)[3]) {
 }
 
 // RHS of every component at this point, visiting the components in the
-// order of visit_comp(); out.put(c, value) is called as soon
-// as component c is final (so its stage update leaves the registers early)
-// and out.done() after the last one.  u[] are the point values.
+// order of visit_comp() (shift, metric, lapse, W, K, connection, A).  Each
+// component's RHS is handed to out.put(c, value) as soon as it is final
+// (so its stage update leaves the registers early); out.done() follows the
+// last one.  On an active Sommerfeld face point the provider reports face()
+// and every component's RHS is replaced by the radial Sommerfeld RHS
+// (reference evolution/boundaries.py:75-172) via out.put_face().
+// u[] are the point values.
 template <typename T, class P, class E>
 BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&u)[NCOMP], E &out) {
   T g[6], A[6], G[3], b[3];
@@ -100,26 +99,28 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
   }
   const T W = (T)u[CW], K = (T)u[CK], al = (T)u[ALPHA];
   const T W2 = W * W;
+  const bool face = pv.face();
   T ig[6];
   inverse_metric(g, ig);
 
-  T R[6], Hs[6], rG[3];
+  T R[6], rG[3];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) { R[c] = T(0); Hs[c] = T(0); }
+  for (int c = 0; c < 6; ++c) R[c] = T(0);
 #pragma unroll
   for (int i = 0; i < 3; ++i) rG[i] = T(0);
 
-  // ---- shift: d_j beta^i, gamma^mn d_m d_n beta^i, 1/3 gamma^im d_m div(beta)
+  // ---- shift (visits 0..2): d_j beta^i; gamma^mn d_m d_n beta^i and
+  //      1/3 gamma^im d_m d_n beta^n (connection terms 8, 9)
   T db[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     pv.stage(i);
+    if (face) out.put_face(BETA + i, (double)pv.somm(BETA + i, u[BETA + i]));
 #pragma unroll
     for (int j = 0; j < 3; ++j) db[i][j] = pv.d1(j);
     T s[6];
     second_derivatives(pv, s);
     rG[i] += contract(ig, s);
-    // d_m d_n beta^n with n = i contributes to the ninth term of every row
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       T acc = ig[sym(r, 0)] * s[sym(0, i)] + ig[sym(r, 1)] * s[sym(1, i)] + ig[sym(r, 2)] * s[sym(2, i)];
@@ -130,64 +131,25 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     out.put(BETA + i, (double)rb);
   }
   const T divb = db[0][0] + db[1][1] + db[2][2];
-
-  // ---- lapse: first derivatives and W^2 d_i d_j alpha
-  T da[3];
-  {
-    pv.stage(3);
+  // Lie terms of A and the shift terms of the connection, while d beta is live
 #pragma unroll
-    for (int j = 0; j < 3; ++j) da[j] = pv.d1(j);
-    T s[6];
-    second_derivatives(pv, s);
+  for (int c = 0; c < 6; ++c) {
+    const int i = SYM_I[c], j = SYM_J[c];
+    T lie = T(0);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) Hs[c] += W2 * s[c];
-    const T slicing = (C.gauge == 0) ? -(al * al) * K : T(-2) * al * K;
-    out.put(ALPHA, (double)(slicing + advection(pv, b) + pv.ko()));
+    for (int m = 0; m < 3; ++m) lie += A[sym(m, i)] * db[m][j] + A[sym(m, j)] * db[m][i];
+    pv.st.put(ST_ALIE + c, (double)(lie - T(2.0 / 3.0) * A[c] * divb));
   }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    rG[i] += T(2.0 / 3.0) * G[i] * divb - (G[0] * db[i][0] + G[1] * db[i][1] + G[2] * db[i][2]);
 
-  // ---- conformal factor: W (d_i d_j W + gamma_ij gamma^mn d_m d_n W)
-  T dW[3];
-  {
-    pv.stage(4);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) dW[j] = pv.d1(j);
-    T s[6];
-    second_derivatives(pv, s);
-    const T lap = contract(ig, s);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) R[c] += W * (s[c] + g[c] * lap);
-    out.put(CW, (double)(advection(pv, b) + T(1.0 / 3.0) * al * W * K - T(1.0 / 3.0) * W * divb + pv.ko()));
-  }
-
-  // ---- trace K: first derivatives
-  T dK[3], rK;
-  {
-    pv.stage(5);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) dK[j] = pv.d1(j);
-    rK = advection(pv, b) + pv.ko();
-  }
-
-  // ---- connection: W^2 (gamma_m(i d_j) Gamma^m)
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    pv.stage(6 + m);
-    T dG[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) dG[j] = pv.d1(j);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const int i = SYM_I[c], j = SYM_J[c];
-      R[c] += W2 * T(0.5) * (g[sym(m, i)] * dG[j] + g[sym(m, j)] * dG[i]);
-    }
-    rG[m] += advection(pv, b) + pv.ko();
-  }
-
-  // ---- conformal metric: d_k gamma_ij and -1/2 W^2 gamma^mn d_m d_n gamma_ij
+  // ---- conformal metric (visits 3..8): d_k gamma_ij, -1/2 W^2 gamma^mn d_m d_n gamma_ij
   T dg[6][3];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
-    pv.stage(9 + c);
+    pv.stage(3 + c);
+    if (face) out.put_face(GXX + c, (double)pv.somm(GXX + c, u[GXX + c]));
 #pragma unroll
     for (int d = 0; d < 3; ++d) dg[c][d] = pv.d1(d);
     T s[6];
@@ -200,16 +162,74 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     out.put(GXX + c, (double)(advection(pv, b) + lie - T(2.0 / 3.0) * g[c] * divb - T(2) * al * A[c] + pv.ko()));
   }
 
-  // ---- traceless A: transport and dissipation
-  T rA[6];
+  // A and the shift are idle until after the Christoffel algebra
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    pv.stage(15 + c);
-    rA[c] = advection(pv, b) + pv.ko();
+  for (int c = 0; c < 6; ++c) pv.st.put(ST_A + c, (double)A[c]);
+
+  // ---- lapse (visit 9): d_i alpha and W^2 d_i d_j alpha
+  T da[3], Hs[6];
+  {
+    pv.stage(9);
+    if (face) out.put_face(ALPHA, (double)pv.somm(ALPHA, u[ALPHA]));
+#pragma unroll
+    for (int j = 0; j < 3; ++j) da[j] = pv.d1(j);
+    T s[6];
+    second_derivatives(pv, s);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) Hs[c] = W2 * s[c];
+    const T slicing = (C.gauge == 0) ? -(al * al) * K : T(-2) * al * K;
+    out.put(ALPHA, (double)(slicing + advection(pv, b) + pv.ko()));
   }
 
-  // ---- Christoffel symbols: C1[i][jk] = Gamma_ijk, C2[i][jk] = Gamma^i_jk
-  T C1[3][6], C2[3][6];
+  // ---- conformal factor (visit 10): W (d_i d_j W + gamma_ij gamma^mn d_m d_n W)
+  T dW[3];
+  {
+    pv.stage(10);
+    if (face) out.put_face(CW, (double)pv.somm(CW, u[CW]));
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dW[j] = pv.d1(j);
+    T s[6];
+    second_derivatives(pv, s);
+    const T lap = contract(ig, s);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) R[c] += W * (s[c] + g[c] * lap);
+    out.put(CW, (double)(advection(pv, b) + T(1.0 / 3.0) * al * W * K - T(1.0 / 3.0) * W * divb + pv.ko()));
+  }
+
+  // ---- trace K (visit 11): -4/3 alpha gamma^ij d_j K goes straight to the connection
+  {
+    pv.stage(11);
+    if (face) out.put_face(CK, (double)pv.somm(CK, u[CK]));
+    T dK[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dK[j] = pv.d1(j);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      rG[i] += T(-4.0 / 3.0) * al * (ig[sym(i, 0)] * dK[0] + ig[sym(i, 1)] * dK[1] + ig[sym(i, 2)] * dK[2]);
+    pv.st.put(ST_RK, (double)(advection(pv, b) + pv.ko()));
+  }
+
+  // ---- connection (visits 12..14): W^2 gamma_m(i d_j) Gamma^m
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    pv.stage(12 + m);
+    if (face) out.put_face(GAM + m, (double)pv.somm(GAM + m, u[GAM + m]));
+    T dG[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dG[j] = pv.d1(j);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const int i = SYM_I[c], j = SYM_J[c];
+      R[c] += W2 * T(0.5) * (g[sym(m, i)] * dG[j] + g[sym(m, j)] * dG[i]);
+    }
+    rG[m] += advection(pv, b) + pv.ko();
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pv.st.put(ST_B + i, (double)b[i]);
+
+  // ---- Christoffel algebra.  C1[i][jk] = Gamma_ijk.  For each upper index k
+  // the row Gamma^k_ij = gamma^km Gamma_mij is formed, used and dropped.
+  T C1[3][6];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
 #pragma unroll
@@ -218,67 +238,55 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
       C1[i][c] = T(0.5) * (dg[sym(i, j)][k] + dg[sym(i, k)][j] - dg[c][i]);
     }
   }
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-      C2[i][c] = ig[sym(i, 0)] * C1[0][c] + ig[sym(i, 1)] * C1[1][c] + ig[sym(i, 2)] * C1[2][c];
-
-  // ---- W^2 R_ij: term_3 + term_4 + term_5 and the Christoffel parts of D_i D_j W.
-  // With Q^{kn}_i = gamma^mn Gamma^k_mi:  term_4_ij = X_ij + X_ji,
-  // X_ij = Q^{kn}_i Gamma_jkn;  term_5_ij = Q^{kn}_i Gamma_kjn.
   {
-    T X[3][3], Y[6];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      T Q[3][3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int n = 0; n < 3; ++n)
-          Q[k][n] = ig[sym(0, n)] * C2[k][sym(0, i)] + ig[sym(1, n)] * C2[k][sym(1, i)] +
-                    ig[sym(2, n)] * C2[k][sym(2, i)];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        T x = T(0), y = T(0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int n = 0; n < 3; ++n) {
-            x += Q[k][n] * C1[j][sym(k, n)];
-            if (j >= i) y += Q[k][n] * C1[k][sym(j, n)];
-          }
-        X[i][j] = x;
-        if (j >= i) Y[sym(i, j)] = y;
-      }
-    }
     const T gradW2 = contract_vec(ig, dW, dW);
-    T conW = T(0);  // Gamma^k d_k W with Gamma^k = gamma^mn Gamma^k_mn
-#pragma unroll
-    for (int k = 0; k < 3; ++k) conW += contract(ig, C2[k]) * dW[k];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const int i = SYM_I[c], j = SYM_J[c];
-      T t3 = T(0);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) t3 += G[m] * (C1[i][sym(j, m)] + C1[j][sym(i, m)]);
-      const T chW = C2[0][c] * dW[0] + C2[1][c] * dW[1] + C2[2][c] * dW[2];
-      R[c] += W2 * (T(0.5) * t3 + (X[i][j] + X[j][i]) + Y[c]) - W * (chW + g[c] * conW) - T(2) * g[c] * gradW2;
-    }
-  }
-
-  // ---- W^2 D_i D_j alpha: Christoffel and gradient parts
-  {
     const T gWa = contract_vec(ig, dW, da);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const int i = SYM_I[c], j = SYM_J[c];
-      const T cha = C2[0][c] * da[0] + C2[1][c] * da[1] + C2[2][c] * da[2];
-      Hs[c] += -W2 * cha + W * (dW[i] * da[j] + dW[j] * da[i] - g[c] * gWa);
+      // W (d_i W d_j alpha + d_j W d_i alpha - gamma_ij gamma^mn d_m W d_n alpha)
+      Hs[c] += W * (dW[i] * da[j] + dW[j] * da[i] - g[c] * gWa);
+      // term_3 = Gamma^m Gamma_(ij)m and -2 gamma_ij |dW|^2
+      T t3 = T(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) t3 += G[m] * (C1[i][sym(j, m)] + C1[j][sym(i, m)]);
+      R[c] += W2 * T(0.5) * t3 - T(2) * g[c] * gradW2;
     }
   }
+  T conW = T(0);  // gamma^mn Gamma^k_mn d_k W
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    T C2k[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      C2k[c] = ig[sym(k, 0)] * C1[0][c] + ig[sym(k, 1)] * C1[1][c] + ig[sym(k, 2)] * C1[2][c];
+    conW += contract(ig, C2k) * dW[k];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      R[c] -= W * C2k[c] * dW[k];
+      Hs[c] -= W2 * C2k[c] * da[k];
+    }
+    // term_4 = gamma^mn (Gamma^k_mi Gamma_jkn + Gamma^k_mj Gamma_ikn), term_5 = gamma^mn Gamma^k_im Gamma_kjn,
+    // one n at a time with Q_i = gamma^mn Gamma^k_mi
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      T Q[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        Q[i] = ig[sym(0, n)] * C2k[sym(0, i)] + ig[sym(1, n)] * C2k[sym(1, i)] + ig[sym(2, n)] * C2k[sym(2, i)];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const int i = SYM_I[c], j = SYM_J[c];
+        R[c] += W2 * (Q[i] * (C1[j][sym(k, n)] + C1[k][sym(j, n)]) + Q[j] * C1[i][sym(k, n)]);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) R[c] -= W * g[c] * conW;
 
-  // ---- A^ij
+  // ---- A^ij and the connection term 2 alpha Gamma^i_jk A^jk = 2 alpha gamma^im Gamma_mjk A^jk
+#pragma unroll
+  for (int c = 0; c < 6; ++c) A[c] = (T)pv.st.get(ST_A + c);
   T Aup[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -291,31 +299,30 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     }
     Aup[c] = acc;
   }
-
-  // ---- trace K
-  out.put(CK, (double)(rK - contract(ig, Hs) + al * contract(A, Aup) + al * K * K * T(1.0 / 3.0)));
-
-  // ---- connection
   {
+    T s1[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) s1[m] = contract(C1[m], Aup);
     const T Wf = W > T(1.0e-12) ? W : T(1.0e-12);
     const T six_al_W = T(6) * al / Wf;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      T t1 = T(0), t3 = T(0), t4 = T(0), t7 = T(0);
+      T t2 = T(0), t3 = T(0), t4 = T(0);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        t1 += ig[sym(i, j)] * dK[j];
+        t2 += ig[sym(i, j)] * s1[j];
         t3 += Aup[sym(i, j)] * dW[j];
         t4 += Aup[sym(i, j)] * da[j];
-        t7 += G[j] * db[i][j];
       }
-      const T t2 = contract(C2[i], Aup);
-      out.put(GAM + i, (double)(rG[i] + T(-4.0 / 3.0) * al * t1 + T(2) * al * t2 - six_al_W * t3 - T(2) * t4 +
-                                T(2.0 / 3.0) * G[i] * divb - t7));
+      out.put(GAM + i, (double)(rG[i] + T(2) * al * t2 - six_al_W * t3 - T(2) * t4));
     }
   }
 
-  // ---- traceless A
+  // ---- trace K
+  out.put(CK, (double)((T)pv.st.get(ST_RK) - contract(ig, Hs) + al * contract(A, Aup) + al * K * K * T(1.0 / 3.0)));
+
+  // ---- everything of the A RHS except transport and dissipation
+  T Arest[6];
   {
     T Xr[6];
 #pragma unroll
@@ -331,12 +338,18 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     for (int c = 0; c < 6; ++c) {
       const int i = SYM_I[c], j = SYM_J[c];
       const T AgA = A[sym(i, 0)] * gA[0][j] + A[sym(i, 1)] * gA[1][j] + A[sym(i, 2)] * gA[2][j];
-      T lie = T(0);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) lie += A[sym(m, i)] * db[m][j] + A[sym(m, j)] * db[m][i];
-      out.put(AXX + c, (double)(rA[c] + al * K * A[c] - T(2) * al * AgA + (Xr[c] - T(1.0 / 3.0) * g[c] * trX) +
-                                lie - T(2.0 / 3.0) * A[c] * divb));
+      Arest[c] = al * K * A[c] - T(2) * al * AgA + (Xr[c] - T(1.0 / 3.0) * g[c] * trX) + (T)pv.st.get(ST_ALIE + c);
     }
+  }
+
+  // ---- traceless A (visits 15..20): transport and dissipation
+#pragma unroll
+  for (int i = 0; i < 3; ++i) b[i] = (T)pv.st.get(ST_B + i);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    pv.stage(15 + c);
+    if (face) out.put_face(AXX + c, (double)pv.somm(AXX + c, u[AXX + c]));
+    out.put(AXX + c, (double)(Arest[c] + advection(pv, b) + pv.ko()));
   }
   out.done();
 }

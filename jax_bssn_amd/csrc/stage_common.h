@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "bssn_amd.h"
@@ -21,7 +22,7 @@ namespace bssn {
 BSSN_HD __forceinline__ void project(double (&g)[6], double (&A)[6]) {
   const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
                      g[2] * (g[1] * g[4] - g[3] * g[2]);
-  const double s = pow(det, -1.0 / 3.0);
+  const double s = 1.0 / cbrt(det);
 #pragma unroll
   for (int c = 0; c < 6; ++c) g[c] *= s;
   double ig[6];
@@ -36,20 +37,30 @@ BSSN_HD __forceinline__ void project(double (&g)[6], double (&A)[6]) {
 // Receives each component's RHS as soon as it is final and applies the
 // stage update of bssn_rk4_stage (include/bssn_amd.h) right away; the
 // metric and A are held until done() so they can be projected together.
+template <class S>
 struct Emitter {
   int64_t n, idx;
-  double dt;
-  const double *u0;
-  double *ustar, *unext, *rhs;
+  double w6, w3, w2, w1;  // dt/6, dt/3, dt/2, dt
+  const BSSN_GLOBAL double *u0;
+  BSSN_GLOBAL double *ustar, *unext, *rhs;
   int mode, reproject;
-  bool active;
-  double gn[6], An[6];
+  bool active, face;
+  S st;  // holds the metric of the stage update until done() (its own copy of the handle)
+  double An[6];
 
-  BSSN_HD Emitter(const Ctx &C, const StageOut &O, int64_t i, bool act)
-      : n(C.ncell), idx(i), dt(C.dt), u0(O.u0), ustar(O.ustar), unext(O.unext), rhs(O.rhs), mode(O.mode),
-        reproject(O.reproject), active(act) {}
+  BSSN_HD __forceinline__ Emitter(const Ctx &C, const StageOut &O, int64_t i, bool act, bool on_face, const S &stash)
+      : n(C.ncell), idx(i), w6(C.dt6), w3(C.dt3), w2(C.dt2), w1(C.dt), u0((const BSSN_GLOBAL double *)O.u0), ustar((BSSN_GLOBAL double *)O.ustar),
+        unext((BSSN_GLOBAL double *)O.unext), rhs((BSSN_GLOBAL double *)O.rhs), mode(O.mode),
+        reproject(O.reproject), active(act), face(on_face), st(stash) {}
 
-  BSSN_HD void put(int c, double k) {
+  // bulk RHS of component c (ignored on a Sommerfeld face point)
+  BSSN_HD __forceinline__ void put(int c, double k) {
+    if (!face) write(c, k);
+  }
+  // the Sommerfeld RHS of component c on a face point
+  BSSN_HD __forceinline__ void put_face(int c, double k) { write(c, k); }
+
+  BSSN_HD __forceinline__ void write(int c, double k) {
     if (!active) return;
     const int64_t at = c * n + idx;
     if (mode == 0) {
@@ -59,17 +70,17 @@ struct Emitter {
     double un;
     if (mode == 1) {
       const double v0 = u0[at];
-      ustar[at] = v0 + (dt / 6.0) * k;
-      un = v0 + (0.5 * dt) * k;
+      ustar[at] = v0 + w6 * k;
+      un = v0 + w2 * k;
     } else if (mode == 4) {
-      un = ustar[at] + (dt / 6.0) * k;
+      un = ustar[at] + w6 * k;
     } else {
       const double v0 = u0[at];
-      ustar[at] = ustar[at] + (dt / 3.0) * k;
-      un = v0 + (mode == 2 ? 0.5 * dt : dt) * k;
+      ustar[at] = ustar[at] + w3 * k;
+      un = v0 + (mode == 2 ? w2 : w1) * k;
     }
     if (c <= GZZ) {
-      gn[c] = un;
+      st.put(ST_GN + c, un);
     } else if (c >= AXX && c <= AZZ) {
       An[c - AXX] = un;
     } else {
@@ -77,8 +88,11 @@ struct Emitter {
     }
   }
 
-  BSSN_HD void done() {
+  BSSN_HD __forceinline__ void done() {
     if (!active || mode == 0) return;
+    double gn[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) gn[c] = st.get(ST_GN + c);
     project(gn, An);
     if (mode == 4 && reproject) project(gn, An);
 #pragma unroll
@@ -119,21 +133,6 @@ struct Slabs {
   int64_t first[7];  // prefix sums of slab sizes
 };
 
-// Sommerfeld RHS of component c (reference evolution/boundaries.py:29-92).
-template <typename T>
-BSSN_HD T sommerfeld_rhs(PointProvider<T> &pv, const Ctx &C, int c, double uc, int x, int y, int z) {
-  pv.stage_comp(c);
-  const T d0 = pv.d1(0), d1 = pv.d1(1), d2 = pv.d1(2);
-  const T X = (T)(C.xmin[0] + C.dx * (double)x);
-  const T Y = (T)(C.xmin[1] + C.dx * (double)y);
-  const T Z = (T)(C.xmin[2] + C.dx * (double)z);
-  const T r = sqrt(X * X + Y * Y + Z * Z);
-  const T rs = r > T(0) ? r : T(1);
-  const bool unit = (c == GXX || c == GYY || c == GZZ || c == CW || c == ALPHA);
-  const T u0 = unit ? T(1) : T(0);
-  const T radial = (X * d0 + Y * d1 + Z * d2) / rs;
-  return -(radial + ((T)uc - u0) / rs);
-}
 
 #define BSSN_FAIL(code, ...) (std::snprintf(err, errlen, __VA_ARGS__), (code))
 
@@ -193,12 +192,19 @@ inline int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C, char *er
   C->g = p->g;
   C->kappa = p->kappa;
   C->dt = p->dt;
+  C->dt6 = p->dt / 6.0;
+  C->dt3 = p->dt / 3.0;
+  C->dt2 = 0.5 * p->dt;
   C->xmin[0] = p->x_min;
   C->xmin[1] = p->y_min;
   C->xmin[2] = p->z_min;
   C->gauge = p->gauge;
   C->zero_shift = p->zero_shift;
   C->any_shell = any_shell;
+  {
+    const char *ord = std::getenv("BSSN_TILE_ORDER");
+    C->tile_order = ord ? std::atoi(ord) : 0;
+  }
   if (p->gauge != 0 && p->gauge != 1) return BSSN_FAIL(BSSN_ERR_INVALID, "gauge must be 0 (harmonic) or 1 (1+log)");
   if (p->kappa != 0.0)
     return BSSN_FAIL(BSSN_ERR_UNSUPPORTED, "kappa != 0 (constraint damping) is not implemented yet");

@@ -24,30 +24,228 @@ namespace bssn {
 // Visit order of the components in the per-point RHS (each visit may
 // restage an LDS tile; the tile provider prefetches visit v + 1 during v).
 __host__ __device__ constexpr int visit_comp(int v) {
-  return v < 3 ? BETA + v : (v == 3 ? ALPHA : (v == 4 ? CW : (v == 5 ? CK : (v < 9 ? GAM + (v - 6) : (v < 15 ? GXX + (v - 9) : AXX + (v - 15))))));
+  // shift 0..2, metric 3..8, lapse 9, W 10, K 11, connection 12..14, A 15..20
+  return v < 3 ? BETA + v
+               : (v < 9 ? GXX + (v - 3)
+                        : (v == 9 ? ALPHA
+                                  : (v == 10 ? CW : (v == 11 ? CK : (v < 15 ? GAM + (v - 12) : AXX + (v - 15))))));
 }
-constexpr int VISIT_FIRST = BETA;
 
 __host__ __device__ inline int wrap_index(int i, int n) {
   int r = i % n;
   return r < 0 ? r + n : r;
 }
 
+// ------------------------------------------------------------ stencil rows
+
+// A one-dimensional stencil row as eight coefficients over the consecutive
+// offsets o0 .. o0+7 from the point (unused taps are zero).  Every row of
+// the reference operators fits: one-sided closures (5, 6 or 8 taps), the
+// centred D4 / D6 / MAD rows and the biased upwind rows.  Coefficients are
+// only ever indexed by constants, so rows live in registers.
+template <typename T>
+struct Row8 {
+  int o0;
+  T c[8];
+};
+
+template <typename T>
+BSSN_HD __forceinline__ void row_clear(Row8<T> &r, int o0) {
+  r.o0 = o0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.c[k] = T(0);
+}
+
+template <typename T>
+BSSN_HD __forceinline__ void row_set5(Row8<T> &r, int o0, double a0, double a1, double a2, double a3, double a4,
+                                      double s) {
+  row_clear(r, o0);
+  r.c[0] = (T)(a0 * s);
+  r.c[1] = (T)(a1 * s);
+  r.c[2] = (T)(a2 * s);
+  r.c[3] = (T)(a3 * s);
+  r.c[4] = (T)(a4 * s);
+}
+
+// Centred first derivative at position p of axis A (reference
+// evolution/derivatives.py:42-162): one-sided closures at the two outer
+// points of an active face, else D4 (or the MAD blend q D4 + (1-q) D6).
+template <typename T>
+BSSN_HD __forceinline__ Row8<T> row_d1(const Axis &A, int p, double inv_dx, bool force_d4) {
+  Row8<T> r;
+  const int s = A.start, e = A.end;
+  if (A.rbc && p == e - 1) {
+    row_set5(r, -4, 1.0 / 4.0, -4.0 / 3.0, 3.0, -4.0, 25.0 / 12.0, inv_dx);
+  } else if (A.rbc && p == e - 2) {
+    row_set5(r, -3, -1.0 / 12.0, 1.0 / 2.0, -3.0 / 2.0, 5.0 / 6.0, 1.0 / 4.0, inv_dx);
+  } else if (A.lbc && p == s + 1) {
+    row_set5(r, -1, -1.0 / 4.0, -5.0 / 6.0, 3.0 / 2.0, -1.0 / 2.0, 1.0 / 12.0, inv_dx);
+  } else if (A.lbc && p == s) {
+    row_set5(r, 0, -25.0 / 12.0, 4.0, -3.0, 4.0 / 3.0, -1.0 / 4.0, inv_dx);
+  } else if (force_d4 || A.q == 1.0) {
+    row_set5(r, -2, 1.0 / 12.0, -2.0 / 3.0, 0.0, 2.0 / 3.0, -1.0 / 12.0, inv_dx);
+  } else {
+    const double q = A.q, w = 1.0 - q;
+    row_clear(r, -3);
+    r.c[0] = (T)(w * (-1.0 / 60.0) * inv_dx);
+    r.c[1] = (T)((q * (1.0 / 12.0) + w * (3.0 / 20.0)) * inv_dx);
+    r.c[2] = (T)((q * (-2.0 / 3.0) + w * (-3.0 / 4.0)) * inv_dx);
+    r.c[4] = (T)((q * (2.0 / 3.0) + w * (3.0 / 4.0)) * inv_dx);
+    r.c[5] = (T)((q * (-1.0 / 12.0) + w * (-3.0 / 20.0)) * inv_dx);
+    r.c[6] = (T)(w * (1.0 / 60.0) * inv_dx);
+  }
+  return r;
+}
+
+// Pure second derivative (reference derivatives.py:341-461).
+template <typename T>
+BSSN_HD __forceinline__ Row8<T> row_d2(const Axis &A, int p, double inv_dx2) {
+  Row8<T> r;
+  const int s = A.start, e = A.end;
+  const double h = inv_dx2;
+  if (A.rbc && p == e - 1) {
+    row_clear(r, -5);
+    const double v[6] = {-5.0 / 6.0, 61.0 / 12.0, -13.0, 107.0 / 6.0, -77.0 / 6.0, 15.0 / 4.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
+  } else if (A.rbc && p == e - 2) {
+    row_clear(r, -4);
+    const double v[6] = {1.0 / 12.0, -1.0 / 2.0, 7.0 / 6.0, -1.0 / 3.0, -5.0 / 4.0, 5.0 / 6.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
+  } else if (A.lbc && p == s + 1) {
+    row_clear(r, -1);
+    const double v[6] = {5.0 / 6.0, -5.0 / 4.0, -1.0 / 3.0, 7.0 / 6.0, -1.0 / 2.0, 1.0 / 12.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
+  } else if (A.lbc && p == s) {
+    row_clear(r, 0);
+    const double v[6] = {15.0 / 4.0, -77.0 / 6.0, 107.0 / 6.0, -13.0, 61.0 / 12.0, -5.0 / 6.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
+  } else if (A.q == 1.0) {
+    row_set5(r, -2, -1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0, h);
+  } else {
+    const double q = A.q, w = 1.0 - q;
+    row_clear(r, -3);
+    const double d4[7] = {0.0, -1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0, 0.0};
+    const double d6[7] = {1.0 / 90.0, -3.0 / 20.0, 3.0 / 2.0, -49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r.c[k] = (T)((q * d4[k] + w * d6[k]) * h);
+  }
+  return r;
+}
+
+// Upwind-biased first derivative (reference derivatives.py:166-338): the
+// sign of the RHS coefficient selects forward (> 0), backward (< 0) or the
+// centred stencil (== 0); within three points of an active face the
+// boundary-aware centred D4 replaces it.
+template <typename T>
+BSSN_HD __forceinline__ Row8<T> row_upwind(const Axis &A, int p, double inv_dx, double coeff) {
+  const int s = A.start, e = A.end;
+  const int wz = A.n < 3 ? A.n : 3;
+  if ((A.lbc && p >= s && p < s + wz) || (A.rbc && p >= e - wz && p < e)) return row_d1<T>(A, p, inv_dx, true);
+  const bool physical = A.lbc || A.rbc;
+  const double q = physical ? 1.0 : A.q, w = 1.0 - q;
+  Row8<T> r;
+  const double s12 = inv_dx * (1.0 / 12.0), s60 = inv_dx * (1.0 / 60.0);
+  if (coeff > 0.0) {
+    // D4 forward on -1..3, D6 forward on -2..4
+    row_clear(r, -2);
+    const double d4[7] = {0.0, -3.0, -10.0, 18.0, -6.0, 1.0, 0.0};
+    const double d6[7] = {2.0, -24.0, -35.0, 80.0, -30.0, 8.0, -1.0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r.c[k] = (T)(q * d4[k] * s12 + w * d6[k] * s60);
+  } else if (coeff < 0.0) {
+    // D4 backward on -3..1, D6 backward on -4..2
+    row_clear(r, -4);
+    const double d4[7] = {0.0, -1.0, 6.0, -18.0, 10.0, 3.0, 0.0};
+    const double d6[7] = {1.0, -8.0, 30.0, -80.0, 35.0, 24.0, -2.0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r.c[k] = (T)(q * d4[k] * s12 + w * d6[k] * s60);
+  } else {
+    Axis B = A;
+    B.q = q;
+    return row_d1<T>(B, p, inv_dx, false);
+  }
+  return r;
+}
+
+// Sixth difference scaled by dx^6 (reference derivatives.py:463-599):
+// 8-point one-sided rows at the three outer points of an active face.
+template <typename T>
+BSSN_HD __forceinline__ Row8<T> row_d6(const Axis &A, int p) {
+  const double L[3][8] = {{4.0, -27.0, 78.0, -125.0, 120.0, -69.0, 22.0, -3.0},
+                          {3.0, -20.0, 57.0, -90.0, 85.0, -48.0, 15.0, -2.0},
+                          {2.0, -13.0, 36.0, -55.0, 50.0, -27.0, 8.0, -1.0}};
+  Row8<T> r;
+  const int s = A.start, e = A.end;
+  const int kr = A.rbc ? e - 1 - p : -1;  // right rows: taps e-1-t -> offsets k - t
+  const int kl = A.lbc ? p - s : -1;      // left rows: taps s+t -> offsets t - k
+  if (kr >= 0 && kr < 3) {
+    row_clear(r, kr - 7);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r.c[j] = (T)(kr == 0 ? L[0][7 - j] : (kr == 1 ? L[1][7 - j] : L[2][7 - j]));
+  } else if (kl >= 0 && kl < 3) {
+    row_clear(r, -kl);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r.c[j] = (T)(kl == 0 ? L[0][j] : (kl == 1 ? L[1][j] : L[2][j]));
+  } else {
+    row_clear(r, -3);
+    const double v[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r.c[k] = (T)v[k];
+  }
+  return r;
+}
+
+// ------------------------------------------------------------------ stash
+
+// Per-thread parking space (slots in bssn_common.h) for values that stay
+// live across the whole point but are idle during the register-heavy
+// Christoffel algebra.
+
+// In LDS, slot-major so that consecutive lanes hit consecutive banks.
+template <int NT>
+struct LdsStash {
+  BSSN_LDS double *base;  // this thread's slot 0 (stride NT between slots)
+  __device__ __forceinline__ void put(int slot, double v) { base[slot * NT] = v; }
+  __device__ double get(int slot) const { return base[slot * NT]; }
+};
+
+// In registers (host build and the shell kernel).
+struct RegStash {
+  double v[STASH_SLOTS];
+  BSSN_HD __forceinline__ void put(int slot, double x) { v[slot] = x; }
+  BSSN_HD __forceinline__ double get(int slot) const { return v[slot]; }
+};
+
 // ---------------------------------------------------------------- LDS tile
 
-template <typename T, int TX, int TY, int TZ, bool MAD>
+// Box halo along an axis of tile extent t: 3 for the centred/upwind/KO
+// stencils, 4 with MAD (sixth-order upwind).  With one-sided closures the
+// taps of a face tile reach 7 points inward (the D6 closure rows), so the
+// box must satisfy t + halo >= 8.
+__host__ __device__ constexpr int tile_halo(int t, bool mad, bool cl) {
+  return (cl && t + (mad ? 4 : 3) < 8) ? 8 - t : (mad ? 4 : 3);
+}
+static_assert(tile_halo(2, false, true) == 6 && tile_halo(4, false, true) == 4 && tile_halo(8, false, true) == 3,
+              "closure halo");
+
+template <typename T, int TX, int TY, int TZ, bool MAD, bool CL>
 struct TileProvider {
-  static constexpr int H = MAD ? 4 : 3;
+  static constexpr int HX = tile_halo(TX, MAD, CL), HY = tile_halo(TY, MAD, CL), HZ = tile_halo(TZ, MAD, CL);
   static constexpr int NT = TX * TY * TZ;
-  static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = TZ + 2 * H;
+  static constexpr int LX = TX + 2 * HX, LY = TY + 2 * HY, LZ = TZ + 2 * HZ;
   static constexpr int SX = LY * LZ, SY = LZ;
   static constexpr int NBOX = LX * LY * LZ;
   static constexpr int NSLOT = (NBOX + NT - 1) / NT;
   static constexpr int LDS_ELEMS = 2 * NBOX;  // double-buffered tile
 
-  T *lds;        // two tile buffers
-  const T *cur;  // buffer of the staged component
-  const double *src;
+  BSSN_LDS T *lds;        // two tile buffers
+  const BSSN_LDS T *cur;  // buffer of the staged component
+  LdsStash<TX * TY * TZ> st;
+  const BSSN_GLOBAL double *src;
   int64_t ncell;
   int goff[NSLOT];
   T pref[NSLOT];  // register prefetch of the next component's tile
@@ -56,49 +254,76 @@ struct TileProvider {
   T inv_dx, inv_dx2, ko_coef;
   T q[3];
   int red[3];
+  // closure support (CL): block-uniform axis flags and the point's position
+  Axis ax[3];
+  int pos[3];
+  int clz[3];     // tile touches a closure zone along the axis
+  bool isface;
+  T xr[3];        // coordinates for the Sommerfeld face RHS
+  double uval;    // point value of the staged component (face RHS)
 
-  __device__ static constexpr int stride(int a) { return a == 0 ? SX : (a == 1 ? SY : 1); }
+  __device__ __forceinline__ static constexpr int stride(int a) { return a == 0 ? SX : (a == 1 ? SY : 1); }
 
-  __device__ void fetch(int comp) {
-    const double *f = src + comp * ncell;
+  __device__ __forceinline__ void fetch(int comp) {
+    const BSSN_GLOBAL double *f = src + comp * ncell;
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) pref[k] = (T)f[goff[k]];
   }
 
-  __device__ void setup(const Ctx &C, const double *u, T *smem, int x0, int y0, int z0,
+  __device__ __forceinline__ void setup(const Ctx &C, const double *u, T *smem, double *stash_mem, int x0, int y0, int z0,
                         int tx, int ty, int tz) {
-    lds = smem;
-    cur = smem;
-    src = u;
+    lds = (BSSN_LDS T *)smem;
+    cur = lds;
+    src = (const BSSN_GLOBAL double *)u;
     ncell = C.ncell;
     tid = (tx * TY + ty) * TZ + tz;
-    c = ((tx + H) * LY + (ty + H)) * LZ + (tz + H);
+    st.base = (BSSN_LDS double *)stash_mem + tid;
+    c = ((tx + HX) * LY + (ty + HY)) * LZ + (tz + HZ);
     inv_dx = (T)C.inv_dx;
     inv_dx2 = (T)C.inv_dx2;
     ko_coef = (T)C.ko;
+    const int o[3] = {x0, y0, z0};
+    const int t[3] = {TX, TY, TZ};
+    const int l[3] = {tx, ty, tz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       q[a] = (T)C.ax[a].q;
       red[a] = C.ax[a].reduced;
+      ax[a] = C.ax[a];
+      pos[a] = o[a] + l[a];
+      clz[a] = CL && ((C.ax[a].lbc && o[a] < C.ax[a].start + 3 && o[a] + t[a] > C.ax[a].start) ||
+                      (C.ax[a].rbc && o[a] + t[a] > C.ax[a].end - 3 && o[a] < C.ax[a].end));
+    }
+    isface = false;
+    if constexpr (CL) {
+      bool f = false, owned = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const Axis &A = C.ax[a];
+        f |= (A.somm_lo && pos[a] == A.start) || (A.somm_hi && pos[a] == A.end - 1);
+        owned &= pos[a] >= A.start && pos[a] < A.end;
+        xr[a] = (T)(C.xmin[a] + C.dx * (double)pos[a]);
+      }
+      isface = f && owned;
     }
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;
       if (e >= NBOX) e = NBOX - 1;
       int lx = e / SX, ly = (e / LZ) % LY, lz = e % LZ;
-      int gx = wrap_index(x0 + lx - H, C.ax[0].n);
-      int gy = wrap_index(y0 + ly - H, C.ax[1].n);
-      int gz = wrap_index(z0 + lz - H, C.ax[2].n);
+      int gx = wrap_index(x0 + lx - HX, C.ax[0].n);
+      int gy = wrap_index(y0 + ly - HY, C.ax[1].n);
+      int gz = wrap_index(z0 + lz - HZ, C.ax[2].n);
       goff[k] = gx * C.ax[0].stride + gy * C.ax[1].stride + gz;
     }
-    fetch(VISIT_FIRST);
+    fetch(visit_comp(0));
   }
 
   // Stage visit v: store the prefetched tile, one barrier, prefetch v + 1.
   // Buffer v & 1 was last read in visit v - 2, which every thread finished
   // before arriving at the barrier of visit v - 1.
-  __device__ void stage(int v) {
-    T *buf = lds + (v & 1) * NBOX;
+  __device__ __forceinline__ void stage(int v) {
+    BSSN_LDS T *buf = lds + (v & 1) * NBOX;
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;
@@ -109,15 +334,35 @@ struct TileProvider {
     cur = buf;
   }
 
-  __device__ T at(int a, int o) const { return cur[c + o * stride(a)]; }
-  __device__ T at2(int a, int oa, int b, int ob) const {
+  __device__ __forceinline__ T at(int a, int o) const { return cur[c + o * stride(a)]; }
+  __device__ __forceinline__ T at2(int a, int oa, int b, int ob) const {
     return cur[c + oa * stride(a) + ob * stride(b)];
   }
-  __device__ T val() const { return cur[c]; }
+  __device__ __forceinline__ T val() const { return cur[c]; }
+
+  // this lane sits in the closure zone of width w along axis a
+  __device__ __forceinline__ bool zone(int a, int w) const {
+    if constexpr (!CL) {
+      return false;
+    } else {
+      if (!clz[a]) return false;
+      const int lo = pos[a] - ax[a].start, hi = ax[a].end - 1 - pos[a];
+      return (ax[a].lbc && lo >= 0 && lo < w) || (ax[a].rbc && hi >= 0 && hi < w);
+    }
+  }
+  __device__ __forceinline__ T apply(int a, const Row8<T> &r) const {
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += r.c[k] * at(a, r.o0 + k);
+    return s;
+  }
+
+  __device__ __forceinline__ bool face() const { return CL && isface; }
 
   // centred first derivative: D4, or q*D4 + (1-q)*D6 with MAD
-  __device__ T d1(int a) const {
+  __device__ __forceinline__ T d1(int a) const {
     if (red[a]) return T(0);
+    if (zone(a, 2)) return apply(a, row_d1<T>(ax[a], pos[a], (double)inv_dx, false));
     T f1 = at(a, 1), b1 = at(a, -1), f2 = at(a, 2), b2 = at(a, -2);
     T d4 = (T(2.0 / 3.0) * (f1 - b1) - T(1.0 / 12.0) * (f2 - b2)) * inv_dx;
     if constexpr (!MAD) {
@@ -129,8 +374,9 @@ struct TileProvider {
     }
   }
 
-  __device__ T d2(int a) const {
+  __device__ __forceinline__ T d2(int a) const {
     if (red[a]) return T(0);
+    if (zone(a, 2)) return apply(a, row_d2<T>(ax[a], pos[a], (double)inv_dx2));
     T f0 = val(), f1 = at(a, 1), b1 = at(a, -1), f2 = at(a, 2), b2 = at(a, -2);
     T d4 = (T(16.0) * (f1 + b1) - (f2 + b2) - T(30.0) * f0) * (inv_dx2 * T(1.0 / 12.0));
     if constexpr (!MAD) {
@@ -144,8 +390,21 @@ struct TileProvider {
   }
 
   // mixed second derivative: inner centred D1 along a, outer along b
-  __device__ T dmix(int a, int b) const {
+  __device__ __forceinline__ T dmix(int a, int b) const {
     if (red[a] || red[b]) return T(0);
+    if (zone(a, 2) || zone(b, 2)) {
+      const Row8<T> ra = row_d1<T>(ax[a], pos[a], (double)inv_dx, false);
+      const Row8<T> rb = row_d1<T>(ax[b], pos[b], (double)inv_dx, false);
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        T inner = T(0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) inner += ra.c[i] * at2(a, ra.o0 + i, b, rb.o0 + j);
+        s += rb.c[j] * inner;
+      }
+      return s;
+    }
     auto inner = [&](int ob) -> T {
       T f1 = at2(a, 1, b, ob), b1 = at2(a, -1, b, ob);
       T f2 = at2(a, 2, b, ob), b2 = at2(a, -2, b, ob);
@@ -170,8 +429,9 @@ struct TileProvider {
   }
 
   // upwind derivative selected by the sign of the RHS coefficient c
-  __device__ T upw(int a, T coeff) const {
+  __device__ __forceinline__ T upw(int a, T coeff) const {
     if (red[a]) return T(0);
+    if (zone(a, 3)) return apply(a, row_upwind<T>(ax[a], pos[a], (double)inv_dx, (double)coeff));
     T f0 = val(), f1 = at(a, 1), b1 = at(a, -1), f2 = at(a, 2), b2 = at(a, -2);
     T f3 = at(a, 3), b3 = at(a, -3);
     T fw = (T(-3.0) * b1 - T(10.0) * f0 + T(18.0) * f1 - T(6.0) * f2 + f3) * (inv_dx * T(1.0 / 12.0));
@@ -193,239 +453,130 @@ struct TileProvider {
   }
 
   // Kreiss-Oliger: nu/64 dx^5 sum_a D6_a
-  __device__ T ko() const {
+  __device__ __forceinline__ T ko() const {
     T f0 = val();
     T s = T(0);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (red[a]) continue;
+      if (zone(a, 3)) {
+        s += apply(a, row_d6<T>(ax[a], pos[a]));
+        continue;
+      }
       T t = (at(a, 3) + at(a, -3)) - T(6.0) * (at(a, 2) + at(a, -2)) + T(15.0) * (at(a, 1) + at(a, -1))
             - T(20.0) * f0;
       s += t;
     }
     return ko_coef * s;
   }
-};
 
-// ------------------------------------------------------- global-memory rows
-
-// A sparse one-dimensional stencil row: value = sum_k c[k] * f[p + off[k]].
-struct Row {
-  int n;
-  int off[8];
-  double c[8];
-  BSSN_HD void push(int o, double v) {
-    for (int k = 0; k < n; ++k)
-      if (off[k] == o) { c[k] += v; return; }
-    off[n] = o;
-    c[n] = v;
-    ++n;
+  // Sommerfeld RHS of the staged component c with point value uc
+  // (reference evolution/boundaries.py:29-92).
+  __device__ __forceinline__ T somm(int comp, double uc) const {
+    const T d0 = d1(0), d1v = d1(1), d2v = d1(2);
+    const T r = sqrt(xr[0] * xr[0] + xr[1] * xr[1] + xr[2] * xr[2]);
+    const T rs = r > T(0) ? r : T(1);
+    const bool unit = (comp == GXX || comp == GYY || comp == GZZ || comp == CW || comp == ALPHA);
+    const T u0 = unit ? T(1) : T(0);
+    return -((xr[0] * d0 + xr[1] * d1v + xr[2] * d2v) / rs + ((T)uc - u0) / rs);
   }
 };
-
-BSSN_HD inline void row_add(Row &r, const int *off, const double *c, int m, double scale) {
-  for (int k = 0; k < m; ++k) r.push(off[k], c[k] * scale);
-}
-
-// Centred D1 row at position p of axis A (closures, MAD).
-BSSN_HD inline Row row_d1(const Axis &A, int p, double inv_dx, bool force_d4) {
-  Row r;
-  r.n = 0;
-  const int s = A.start, e = A.end;
-  if (A.rbc && p == e - 1) {
-    const int o[5] = {-4, -3, -2, -1, 0};
-    const double c[5] = {1.0 / 4.0, -4.0 / 3.0, 3.0, -4.0, 25.0 / 12.0};
-    row_add(r, o, c, 5, inv_dx);
-  } else if (A.rbc && p == e - 2) {
-    const int o[5] = {-3, -2, -1, 0, 1};
-    const double c[5] = {-1.0 / 12.0, 1.0 / 2.0, -3.0 / 2.0, 5.0 / 6.0, 1.0 / 4.0};
-    row_add(r, o, c, 5, inv_dx);
-  } else if (A.lbc && p == s + 1) {
-    const int o[5] = {-1, 0, 1, 2, 3};
-    const double c[5] = {-1.0 / 4.0, -5.0 / 6.0, 3.0 / 2.0, -1.0 / 2.0, 1.0 / 12.0};
-    row_add(r, o, c, 5, inv_dx);
-  } else if (A.lbc && p == s) {
-    const int o[5] = {0, 1, 2, 3, 4};
-    const double c[5] = {-25.0 / 12.0, 4.0, -3.0, 4.0 / 3.0, -1.0 / 4.0};
-    row_add(r, o, c, 5, inv_dx);
-  } else {
-    const int o4[4] = {-2, -1, 1, 2};
-    const double c4[4] = {1.0 / 12.0, -2.0 / 3.0, 2.0 / 3.0, -1.0 / 12.0};
-    if (force_d4 || A.q == 1.0) {
-      row_add(r, o4, c4, 4, inv_dx);
-    } else {
-      const int o6[6] = {-3, -2, -1, 1, 2, 3};
-      const double c6[6] = {-1.0 / 60.0, 3.0 / 20.0, -3.0 / 4.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
-      row_add(r, o4, c4, 4, A.q * inv_dx);
-      row_add(r, o6, c6, 6, (1.0 - A.q) * inv_dx);
-    }
-  }
-  return r;
-}
-
-BSSN_HD inline Row row_d2(const Axis &A, int p, double inv_dx2) {
-  Row r;
-  r.n = 0;
-  const int s = A.start, e = A.end;
-  if (A.rbc && p == e - 1) {
-    const int o[6] = {-5, -4, -3, -2, -1, 0};
-    const double c[6] = {-5.0 / 6.0, 61.0 / 12.0, -13.0, 107.0 / 6.0, -77.0 / 6.0, 15.0 / 4.0};
-    row_add(r, o, c, 6, inv_dx2);
-  } else if (A.rbc && p == e - 2) {
-    const int o[6] = {-4, -3, -2, -1, 0, 1};
-    const double c[6] = {1.0 / 12.0, -1.0 / 2.0, 7.0 / 6.0, -1.0 / 3.0, -5.0 / 4.0, 5.0 / 6.0};
-    row_add(r, o, c, 6, inv_dx2);
-  } else if (A.lbc && p == s + 1) {
-    const int o[6] = {-1, 0, 1, 2, 3, 4};
-    const double c[6] = {5.0 / 6.0, -5.0 / 4.0, -1.0 / 3.0, 7.0 / 6.0, -1.0 / 2.0, 1.0 / 12.0};
-    row_add(r, o, c, 6, inv_dx2);
-  } else if (A.lbc && p == s) {
-    const int o[6] = {0, 1, 2, 3, 4, 5};
-    const double c[6] = {15.0 / 4.0, -77.0 / 6.0, 107.0 / 6.0, -13.0, 61.0 / 12.0, -5.0 / 6.0};
-    row_add(r, o, c, 6, inv_dx2);
-  } else {
-    const int o4[5] = {-2, -1, 0, 1, 2};
-    const double c4[5] = {-1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0};
-    if (A.q == 1.0) {
-      row_add(r, o4, c4, 5, inv_dx2);
-    } else {
-      const int o6[7] = {-3, -2, -1, 0, 1, 2, 3};
-      const double c6[7] = {1.0 / 90.0, -3.0 / 20.0, 3.0 / 2.0, -49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
-      row_add(r, o4, c4, 5, A.q * inv_dx2);
-      row_add(r, o6, c6, 7, (1.0 - A.q) * inv_dx2);
-    }
-  }
-  return r;
-}
-
-BSSN_HD inline Row row_upwind(const Axis &A, int p, double inv_dx, double coeff) {
-  const int s = A.start, e = A.end;
-  const int w = A.n < 3 ? A.n : 3;
-  if ((A.lbc && p >= s && p < s + w) || (A.rbc && p >= e - w && p < e))
-    return row_d1(A, p, inv_dx, true);
-  Row r;
-  r.n = 0;
-  const bool physical = A.lbc || A.rbc;
-  const double q = physical ? 1.0 : A.q;
-  const double s12 = inv_dx / 12.0, s60 = inv_dx / 60.0;
-  if (coeff > 0.0) {
-    const int o[5] = {-1, 0, 1, 2, 3};
-    const double c[5] = {-3.0, -10.0, 18.0, -6.0, 1.0};
-    row_add(r, o, c, 5, q * s12);
-    if (q != 1.0) {
-      const int o6[7] = {-2, -1, 0, 1, 2, 3, 4};
-      const double c6[7] = {2.0, -24.0, -35.0, 80.0, -30.0, 8.0, -1.0};
-      row_add(r, o6, c6, 7, (1.0 - q) * s60);
-    }
-  } else if (coeff < 0.0) {
-    const int o[5] = {-3, -2, -1, 0, 1};
-    const double c[5] = {-1.0, 6.0, -18.0, 10.0, 3.0};
-    row_add(r, o, c, 5, q * s12);
-    if (q != 1.0) {
-      const int o6[7] = {-4, -3, -2, -1, 0, 1, 2};
-      const double c6[7] = {1.0, -8.0, 30.0, -80.0, 35.0, 24.0, -2.0};
-      row_add(r, o6, c6, 7, (1.0 - q) * s60);
-    }
-  } else {
-    Axis B = A;
-    B.q = q;
-    return row_d1(B, p, inv_dx, false);
-  }
-  return r;
-}
-
-// Sixth difference row scaled by dx^6 (i.e. the bare integer stencil).
-BSSN_HD inline Row row_d6(const Axis &A, int p) {
-  const double L[3][8] = {{4.0, -27.0, 78.0, -125.0, 120.0, -69.0, 22.0, -3.0},
-                          {3.0, -20.0, 57.0, -90.0, 85.0, -48.0, 15.0, -2.0},
-                          {2.0, -13.0, 36.0, -55.0, 50.0, -27.0, 8.0, -1.0}};
-  Row r;
-  r.n = 0;
-  const int s = A.start, e = A.end;
-  for (int k = 0; k < 3; ++k) {
-    if (A.rbc && p == e - 1 - k) {
-      for (int t = 0; t < 8; ++t) r.push(k - t, L[k][t]);  // taps e-1-t of the point e-1-k
-      return r;
-    }
-  }
-  for (int k = 0; k < 3; ++k) {
-    if (A.lbc && p == s + k) {
-      for (int t = 0; t < 8; ++t) r.push(t - k, L[k][t]);
-      return r;
-    }
-  }
-  const int o[7] = {-3, -2, -1, 0, 1, 2, 3};
-  const double c[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
-  row_add(r, o, c, 7, 1.0);
-  return r;
-}
 
 template <typename T>
 struct PointProvider {
+  RegStash st;
   const Ctx *C;
-  const double *src;
-  const double *f;
+  const BSSN_GLOBAL double *src;
+  const BSSN_GLOBAL double *f;
   int p[3];
   int64_t idx;
 
-  BSSN_HD void setup(const Ctx &ctx, const double *u, int x, int y, int z) {
+  bool isface;
+
+  BSSN_HD __forceinline__ void setup(const Ctx &ctx, const double *u, int x, int y, int z) {
     C = &ctx;
-    src = u;
-    f = u;
+    src = (const BSSN_GLOBAL double *)u;
+    f = src;
     p[0] = x; p[1] = y; p[2] = z;
     idx = (int64_t)x * ctx.ax[0].stride + (int64_t)y * ctx.ax[1].stride + z;
+    bool fc = false, owned = true;
+    for (int a = 0; a < 3; ++a) {
+      const Axis &A = ctx.ax[a];
+      fc |= (A.somm_lo && p[a] == A.start) || (A.somm_hi && p[a] == A.end - 1);
+      owned &= p[a] >= A.start && p[a] < A.end;
+    }
+    isface = fc && owned;
   }
-  BSSN_HD void stage(int v) { f = src + visit_comp(v) * C->ncell; }
-  BSSN_HD void stage_comp(int comp) { f = src + comp * C->ncell; }
+  BSSN_HD __forceinline__ bool face() const { return isface; }
+  BSSN_HD __forceinline__ void stage(int v) { f = src + visit_comp(v) * C->ncell; }
+  BSSN_HD __forceinline__ void stage_comp(int comp) { f = src + comp * C->ncell; }
 
-  BSSN_HD T ld(int a, int oa) const {
+  BSSN_HD __forceinline__ T ld(int a, int oa) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
     return (T)f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]];
   }
-  BSSN_HD T ld2(int a, int oa, int b, int ob) const {
+  BSSN_HD __forceinline__ T ld2(int a, int oa, int b, int ob) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
     q[b] = wrap_index(q[b] + ob, C->ax[b].n);
     return (T)f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]];
   }
-  BSSN_HD T apply(int a, const Row &r) const {
+  BSSN_HD __forceinline__ T apply(int a, const Row8<T> &r) const {
     T s = T(0);
-    for (int k = 0; k < r.n; ++k) s += (T)r.c[k] * ld(a, r.off[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (r.c[k] != T(0)) s += r.c[k] * ld(a, r.o0 + k);
     return s;
   }
-  BSSN_HD T val() const { return (T)f[idx]; }
-  BSSN_HD T d1(int a) const {
+  BSSN_HD __forceinline__ T val() const { return (T)f[idx]; }
+  BSSN_HD __forceinline__ T d1(int a) const {
     if (C->ax[a].reduced) return T(0);
-    return apply(a, row_d1(C->ax[a], p[a], C->inv_dx, false));
+    return apply(a, row_d1<T>(C->ax[a], p[a], C->inv_dx, false));
   }
-  BSSN_HD T d2(int a) const {
+  BSSN_HD __forceinline__ T d2(int a) const {
     if (C->ax[a].reduced) return T(0);
-    return apply(a, row_d2(C->ax[a], p[a], C->inv_dx2));
+    return apply(a, row_d2<T>(C->ax[a], p[a], C->inv_dx2));
   }
-  BSSN_HD T dmix(int a, int b) const {
+  BSSN_HD __forceinline__ T dmix(int a, int b) const {
     if (C->ax[a].reduced || C->ax[b].reduced) return T(0);
-    Row ra = row_d1(C->ax[a], p[a], C->inv_dx, false);
-    Row rb = row_d1(C->ax[b], p[b], C->inv_dx, false);
+    const Row8<T> ra = row_d1<T>(C->ax[a], p[a], C->inv_dx, false);
+    const Row8<T> rb = row_d1<T>(C->ax[b], p[b], C->inv_dx, false);
     T s = T(0);
-    for (int j = 0; j < rb.n; ++j) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (rb.c[j] == T(0)) continue;
       T inner = T(0);
-      for (int i = 0; i < ra.n; ++i) inner += (T)ra.c[i] * ld2(a, ra.off[i], b, rb.off[j]);
-      s += (T)rb.c[j] * inner;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (ra.c[i] != T(0)) inner += ra.c[i] * ld2(a, ra.o0 + i, b, rb.o0 + j);
+      s += rb.c[j] * inner;
     }
     return s;
   }
-  BSSN_HD T upw(int a, T coeff) const {
+  BSSN_HD __forceinline__ T upw(int a, T coeff) const {
     if (C->ax[a].reduced) return T(0);
-    return apply(a, row_upwind(C->ax[a], p[a], C->inv_dx, (double)coeff));
+    return apply(a, row_upwind<T>(C->ax[a], p[a], C->inv_dx, (double)coeff));
   }
-  BSSN_HD T ko() const {
+  BSSN_HD __forceinline__ T ko() const {
     T s = T(0);
     for (int a = 0; a < 3; ++a) {
       if (C->ax[a].reduced) continue;
-      s += apply(a, row_d6(C->ax[a], p[a]));
+      s += apply(a, row_d6<T>(C->ax[a], p[a]));
     }
     return (T)C->ko * s;
+  }
+  // Sommerfeld RHS of the staged component (reference evolution/boundaries.py:29-92)
+  BSSN_HD __forceinline__ T somm(int comp, double uc) const {
+    const T d0 = d1(0), d1v = d1(1), d2v = d1(2);
+    const T X = (T)(C->xmin[0] + C->dx * (double)p[0]);
+    const T Y = (T)(C->xmin[1] + C->dx * (double)p[1]);
+    const T Z = (T)(C->xmin[2] + C->dx * (double)p[2]);
+    const T r = sqrt(X * X + Y * Y + Z * Z);
+    const T rs = r > T(0) ? r : T(1);
+    const bool unit = (comp == GXX || comp == GYY || comp == GZZ || comp == CW || comp == ALPHA);
+    const T u0 = unit ? T(1) : T(0);
+    return -((X * d0 + Y * d1v + Z * d2v) / rs + ((T)uc - u0) / rs);
   }
 };
 

@@ -23,13 +23,8 @@ int run(const bssn_params *p, const bssn_grid *g, int mode, const double *us, co
         pv.setup(C, us, x, y, z);
         double u[NCOMP];
         for (int c = 0; c < NCOMP; ++c) u[c] = us[c * C.ncell + pv.idx];
-        Emitter out(C, O, pv.idx, true);
-        if (sommerfeld_point(C, x, y, z)) {
-          for (int c = 0; c < NCOMP; ++c) out.put(c, sommerfeld_rhs<double>(pv, C, c, u[c], x, y, z));
-          out.done();
-        } else {
-          bssn_rhs_point<double>(pv, C, u, out);
-        }
+        Emitter<RegStash> out(C, O, pv.idx, true, pv.face(), pv.st);
+        bssn_rhs_point<double>(pv, C, u, out);
       }
   return 0;
 }
