@@ -83,7 +83,8 @@ def tile_kernel_roofline(J, ev, precision, reps=3):
         _lib.check(lib.bssn_time_stage_part(ctypes.byref(cp), ctypes.byref(cg), prec, s, 1,
                                             ctypes.c_void_p(us.data_ptr()), ctypes.c_void_p(u0.data_ptr()),
                                             ctypes.c_void_p(ustar.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                                            reps, ctypes.byref(ms), st))
+                                            reps, ctypes.byref(ms), ctypes.c_void_p(ev.stage_scratch.data_ptr()),
+                                            st))
         per_stage[s] = ms.value
         total_bytes += BYTES_PER_CELL[s] * npts
         total_s += ms.value / 1e3

@@ -73,10 +73,25 @@ const char *bssn_last_error_string(void);
 /* Library version string ("bssn_amd <semver> gfx950"). */
 const char *bssn_version(void);
 
+/* Caller-owned scratch, in doubles: what = 0 for bssn_rhs / bssn_rk4_stage
+ * (non-zero only when kappa != 0: the derived grid functions of the
+ * constraint-damping term), what = 1 for bssn_evolve (3 state-sized RK4
+ * buffers plus the former). */
+int64_t bssn_work_size(const bssn_params *p, const bssn_grid *grid, int what);
+
 /* Replaces reference evolution/time_evolve.py:41-57 compute_bssn_rhs.
- * rhs[21*n] <- RHS(state[21*n]); state and rhs must not alias. */
+ * rhs[21*n] <- RHS(state[21*n]); state and rhs must not alias.  work holds
+ * bssn_work_size(p, grid, 0) doubles (may be NULL when that is 0). */
 int bssn_rhs(const bssn_params *p, const bssn_grid *grid, int precision,
-             const double *state, double *rhs, void *stream);
+             const double *state, double *rhs, double *work, void *stream);
+
+/* Replaces reference time_evolve.py:60-91 compute_bssn_rhs_with_matter:
+ * matter[10*n] = rho, S_x, S_y, S_z, S_xx, S_xy, S_xz, S_yy, S_yz, S_zz
+ * (reference layouts rho[n], S_i[3,n], S_ij[3,3,n] with S_ij stored once).
+ * Other arguments as bssn_rhs. */
+int bssn_rhs_with_matter(const bssn_params *p, const bssn_grid *grid, int precision,
+                         const double *state, const double *matter, double *rhs,
+                         double *work, void *stream);
 
 /* Replaces reference time_evolve.py:134-141 enforce_algebraic_constraints:
  * out <- P(in) pointwise (det = 1 rescale, then trace-free A).  in == out
@@ -94,14 +109,15 @@ int bssn_enforce_constraints(const bssn_grid *grid, const double *in,
  *                                    (P(P(...)) when reproject != 0)
  * u_next must alias neither u_s nor u0 (except stage 4, where u_next ==
  * u0 is allowed); u0 is unused in stage 4; ustar is unused in stage 4's
- * write set. */
+ * write set.  work as for bssn_rhs. */
 int bssn_rk4_stage(const bssn_params *p, const bssn_grid *grid, int precision,
                    int stage, const double *u_s, const double *u0,
-                   double *ustar, double *u_next, int reproject, void *stream);
+                   double *ustar, double *u_next, int reproject, double *work,
+                   void *stream);
 
 /* Replaces reference time_evolve.py:144-217 rk4_step applied nsteps times:
- * state <- rk4_step^nsteps(state), in place.  work must hold 3*21*n
- * doubles (caller-owned; no allocation).  Identical to calling rk4_step
+ * state <- rk4_step^nsteps(state), in place.  work must hold
+ * bssn_work_size(p, grid, 1) doubles (caller-owned; no allocation).  Identical to calling rk4_step
  * nsteps times (the projection at the start of each step included). */
 int bssn_evolve(const bssn_params *p, const bssn_grid *grid, int precision,
                 int nsteps, double *state, double *work, void *stream);
@@ -110,6 +126,13 @@ int bssn_evolve(const bssn_params *p, const bssn_grid *grid, int precision,
  * out[9*n] = H, M_x, M_y, M_z, det-1, tr A, G^x, G^y, G^z (each n points). */
 int bssn_constraints(const bssn_params *p, const bssn_grid *grid,
                      const double *state, double *out, void *stream);
+
+/* Replaces reference bssn/constraints.py:415-440: as bssn_constraints with
+ * H - 16 pi rho and M_i - 8 pi S_i (matter layout as bssn_rhs_with_matter,
+ * NULL = vacuum). */
+int bssn_constraints_with_matter(const bssn_params *p, const bssn_grid *grid,
+                                 const double *state, const double *matter,
+                                 double *out, void *stream);
 
 /* Replaces reference diagnostics/constraints.py:9-36 compute_constraint_norms
  * over the constraint fields bssn_constraints writes.
@@ -127,7 +150,7 @@ int bssn_constraint_norms(const bssn_grid *grid, const double *constraints,
  * stores the mean duration per launch in *ms.  Buffers as bssn_rk4_stage. */
 int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precision, int stage, int part,
                          const double *u_s, const double *u0, double *ustar, double *u_next, int reps,
-                         float *ms, void *stream);
+                         float *ms, double *work, void *stream);
 
 /* Number of grid points part 1 (tile kernel) or part 2 (shell kernel) of a
  * stage updates (negative status on invalid input). */

@@ -13,6 +13,7 @@ from .variables import BSSNParameters, BSSNVariables, get_boundary_codes, pack, 
 from .time_evolve import (  # noqa: F401
     Evolver,
     compute_bssn_rhs,
+    compute_bssn_rhs_with_matter,
     enforce_algebraic_constraints,
     evolve,
     evolve_packed,
@@ -22,6 +23,7 @@ from .time_evolve import (  # noqa: F401
 from .constraints import (  # noqa: F401
     ConstraintViolations,
     compute_all_constraints,
+    compute_all_constraints_with_matter,
     compute_constraint_norms,
     compute_hamiltonian_constraint,
 )

@@ -43,24 +43,31 @@ class CGrid(ctypes.Structure):
 _SIGNATURES = {
     "bssn_last_error_string": (ctypes.c_char_p, []),
     "bssn_version": (ctypes.c_char_p, []),
+    "bssn_work_size": (ctypes.c_int64, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int]),
     "bssn_rhs": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
-                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "bssn_rhs_with_matter": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
     "bssn_enforce_constraints": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_void_p,
                                                 ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_rk4_stage": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_evolve": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
                                    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_constraints": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid),
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "bssn_constraints_with_matter": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid),
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p]),
     "bssn_norm_partial_size": (ctypes.c_int64, [ctypes.POINTER(CGrid)]),
     "bssn_constraint_norms": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_time_stage_part": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
-                                            ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
+                                            ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_stage_points": (ctypes.c_int64, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int]),
 }
 
@@ -124,6 +131,14 @@ def c_params(params):
         for i in range(3):
             p.reduced_axes[i] = 1 if params.reduced_axes[i] else 0
     return p
+
+
+def work_size(params, shape, what):
+    """Doubles of caller-owned scratch (bssn_work_size)."""
+    n = load().bssn_work_size(ctypes.byref(c_params(params)), ctypes.byref(c_grid(shape)), int(what))
+    if n < 0:
+        check(int(n))
+    return int(n)
 
 
 def c_grid(shape):

@@ -25,13 +25,14 @@ NORM_NAMES = ("hamiltonian_l2", "momentum_l2", "det_gamma_l2", "trace_A_l2", "ga
               "hamiltonian_linf", "momentum_linf", "det_gamma_linf", "trace_A_linf", "gamma_linf")
 
 
-def constraints_packed(state, params, out=None):
+def constraints_packed(state, params, out=None, matter=None):
     """(9, nx, ny, nz): H, M_x, M_y, M_z, det-1, tr A, G^x, G^y, G^z."""
     lib = _lib.load()
     if out is None:
         out = torch.empty((9,) + tuple(state.shape[1:]), dtype=torch.float64, device=state.device)
-    _lib.check(lib.bssn_constraints(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(state.shape[1:])),
-                                    _ptr(state), _ptr(out), stream_handle()))
+    _lib.check(lib.bssn_constraints_with_matter(
+        ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(state.shape[1:])), _ptr(state),
+        _ptr(matter) if matter is not None else None, _ptr(out), stream_handle()))
     return out
 
 
@@ -56,6 +57,16 @@ def _split(cf, like):
 def compute_all_constraints(vars, params):
     """Reference bssn/constraints.py:387-412."""
     return _split(constraints_packed(pack(vars), params), _like(vars))
+
+
+def compute_all_constraints_with_matter(vars, params, energy_density, momentum_density):
+    """Reference bssn/constraints.py:415-440."""
+    from .time_evolve import pack_matter
+    state = pack(vars)
+    shape = tuple(state.shape[1:])
+    zeros = torch.zeros((3, 3) + shape, dtype=torch.float64)
+    matter = pack_matter(energy_density, momentum_density, zeros, state.device)
+    return _split(constraints_packed(state, params, matter=matter), _like(vars))
 
 
 def compute_hamiltonian_constraint(vars, params):

@@ -160,7 +160,8 @@ __global__ void __launch_bounds__(256) k_project(int64_t n, const double *__rest
 
 // Reference bssn/constraints.py:22-97 (momentum), 271-303 (Hamiltonian),
 // 318-350 (det, trace), 353-384 (Gamma constraint).
-__global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__restrict__ us, double *out) {
+__global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__restrict__ us,
+                                                     const double *__restrict__ matter, double *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= C.ncell) return;
   const int z = (int)(i % C.ax[2].n);
@@ -248,7 +249,8 @@ __global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__rest
     Aup[c] = acc;
   }
   const int64_t n = C.ncell;
-  out[0 * n + i] = contract(ig, R) + 2.0 / 3.0 * K * K - contract(A, Aup);
+  out[0 * n + i] = contract(ig, R) + 2.0 / 3.0 * K * K - contract(A, Aup) -
+                   (matter ? 16.0 * M_PI * matter[i] : 0.0);
 
   // momentum constraint
   const double Wf = W > 1.0e-12 ? W : 1.0e-12;
@@ -285,13 +287,27 @@ __global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__rest
     }
     double second = 0.0;
     for (int c = 0; c < 6; ++c) second += (SYM_I[c] == SYM_J[c] ? 1.0 : 2.0) * ig[c] * dA[c][r];
-    out[(1 + r) * n + i] = first - 0.5 * second - 3.0 * Bw / Wf - 2.0 / 3.0 * dK[r];
+    out[(1 + r) * n + i] = first - 0.5 * second - 3.0 * Bw / Wf - 2.0 / 3.0 * dK[r] -
+                           (matter ? 8.0 * M_PI * matter[(1 + r) * n + i] : 0.0);
   }
   const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
                      g[2] * (g[1] * g[4] - g[3] * g[2]);
   out[4 * n + i] = det - 1.0;
   out[5 * n + i] = contract(ig, A);
   for (int r = 0; r < 3; ++r) out[(6 + r) * n + i] = G[r] - contract(ig, C2[r]);
+}
+
+// ------------------------------------------------- constraint damping
+
+__global__ void __launch_bounds__(256) k_derived(int64_t n, const double *__restrict__ us, double *D) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) derived_point(n, i, us, D);
+}
+
+__global__ void __launch_bounds__(256) k_kappa(Ctx C, const double *__restrict__ us, const double *__restrict__ D,
+                                               const double *__restrict__ matter, double *kap) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < C.ncell) kappa_point(C, us, D, matter, i, kap);
 }
 
 // ------------------------------------------------------------- reductions
@@ -448,7 +464,30 @@ int launch_stage_p(const Ctx &C, const double *us, const StageOut &O, hipStream_
   return cl ? launch_stage_t<T, false, true>(C, us, O, st, parts) : launch_stage_t<T, false, false>(C, us, O, st, parts);
 }
 
-int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &O, hipStream_t st, int parts = 3) {
+// Per-stage scratch (doubles) beyond the state buffers: the derived grid
+// functions and the kappa term when kappa != 0.
+int64_t stage_scratch(const Ctx &C) { return C.kappa != 0.0 ? (int64_t)(NDERIVED + 6) * C.ncell : 0; }
+
+int launch_kappa(const Ctx &C, const double *us, const double *matter, double *work, hipStream_t st) {
+  const int64_t blocks = (C.ncell + 255) / 256;
+  double *D = work, *kap = work + (int64_t)NDERIVED * C.ncell;
+  hipLaunchKernelGGL(k_derived, dim3((unsigned)blocks), dim3(256), 0, st, C.ncell, us, D);
+  int rc = check_hip(hipGetLastError(), "derived-field kernel launch");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_kappa, dim3((unsigned)blocks), dim3(256), 0, st, C, us, D, matter, kap);
+  return check_hip(hipGetLastError(), "kappa kernel launch");
+}
+
+int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &O0, hipStream_t st, int parts = 3,
+                 double *work = nullptr) {
+  StageOut O = O0;
+  stage_update(O, C.dt);
+  if (C.kappa != 0.0) {
+    if (!work) return fail(BSSN_ERR_INVALID, "kappa != 0 needs a work buffer of bssn_work_size() doubles");
+    const int rc = launch_kappa(C, us, O.matter, work, st);
+    if (rc) return rc;
+    O.kap = work + (int64_t)NDERIVED * C.ncell;
+  }
   if (precision == BSSN_PREC_F64) return launch_stage_p<double>(C, us, O, st, parts);
   if (precision == BSSN_PREC_F32RHS) return launch_stage_p<float>(C, us, O, st, parts);
   return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
@@ -480,15 +519,35 @@ const char *bssn_last_error_string(void) { return g_error.c_str(); }
 
 const char *bssn_version(void) { return "bssn_amd 0.1.0 gfx950"; }
 
+int64_t bssn_work_size(const bssn_params *p, const bssn_grid *grid, int what) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (what == 0) return stage_scratch(C);
+  if (what == 1) return 3 * NCOMP * C.ncell + stage_scratch(C);
+  return fail(BSSN_ERR_INVALID, "what must be 0 (bssn_rhs / bssn_rk4_stage) or 1 (bssn_evolve)");
+}
+
 int bssn_rhs(const bssn_params *p, const bssn_grid *grid, int precision, const double *state, double *rhs,
-             void *stream) {
+             double *work, void *stream) {
   Ctx C;
   int rc = resolve(p, grid, &C);
   if (rc) return rc;
   if (!state || !rhs) return fail(BSSN_ERR_INVALID, "null state or rhs pointer");
   if (state == rhs) return fail(BSSN_ERR_INVALID, "state and rhs must not alias");
-  StageOut O{nullptr, nullptr, nullptr, rhs, 0, 0};
-  return launch_stage(C, precision, state, O, (hipStream_t)stream);
+  StageOut O{nullptr, nullptr, nullptr, rhs, 0, 0, nullptr, nullptr};
+  return launch_stage(C, precision, state, O, (hipStream_t)stream, 3, work);
+}
+
+int bssn_rhs_with_matter(const bssn_params *p, const bssn_grid *grid, int precision, const double *state,
+                         const double *matter, double *rhs, double *work, void *stream) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (!state || !rhs || !matter) return fail(BSSN_ERR_INVALID, "null state, matter or rhs pointer");
+  if (state == rhs) return fail(BSSN_ERR_INVALID, "state and rhs must not alias");
+  StageOut O{nullptr, nullptr, nullptr, rhs, 0, 0, nullptr, matter};
+  return launch_stage(C, precision, state, O, (hipStream_t)stream, 3, work);
 }
 
 int bssn_enforce_constraints(const bssn_grid *grid, const double *in, double *out, void *stream) {
@@ -499,7 +558,7 @@ int bssn_enforce_constraints(const bssn_grid *grid, const double *in, double *ou
 }
 
 int bssn_rk4_stage(const bssn_params *p, const bssn_grid *grid, int precision, int stage, const double *u_s,
-                   const double *u0, double *ustar, double *u_next, int reproject, void *stream) {
+                   const double *u0, double *ustar, double *u_next, int reproject, double *work, void *stream) {
   Ctx C;
   int rc = resolve(p, grid, &C);
   if (rc) return rc;
@@ -507,8 +566,8 @@ int bssn_rk4_stage(const bssn_params *p, const bssn_grid *grid, int precision, i
   if (!u_s || !ustar || !u_next || (stage != 4 && !u0)) return fail(BSSN_ERR_INVALID, "null buffer");
   if (u_next == u_s) return fail(BSSN_ERR_INVALID, "u_next must not alias u_s");
   if (stage != 4 && u_next == u0) return fail(BSSN_ERR_INVALID, "u_next must not alias u0 before stage 4");
-  StageOut O{stage == 1 ? u_s : u0, ustar, u_next, nullptr, stage, reproject};
-  return launch_stage(C, precision, u_s, O, (hipStream_t)stream);
+  StageOut O{stage == 1 ? u_s : u0, ustar, u_next, nullptr, stage, reproject, nullptr, nullptr};
+  return launch_stage(C, precision, u_s, O, (hipStream_t)stream, 3, work);
 }
 
 int bssn_evolve(const bssn_params *p, const bssn_grid *grid, int precision, int nsteps, double *state,
@@ -521,24 +580,25 @@ int bssn_evolve(const bssn_params *p, const bssn_grid *grid, int precision, int 
   if (nsteps == 0) return BSSN_OK;
   hipStream_t st = (hipStream_t)stream;
   const int64_t n = C.ncell * NCOMP;
-  double *ustar = work, *ua = work + n, *ub = work + 2 * n;
+  double *ustar = work, *ua = work + n, *ub = work + 2 * n, *scratch = work + 3 * n;
   rc = launch_project(C.ncell, state, state, st);
   if (rc) return rc;
   for (int s = 0; s < nsteps; ++s) {
     const int last = s == nsteps - 1;
-    StageOut O1{state, ustar, ua, nullptr, 1, 0};
-    if ((rc = launch_stage(C, precision, state, O1, st))) return rc;
-    StageOut O2{state, ustar, ub, nullptr, 2, 0};
-    if ((rc = launch_stage(C, precision, ua, O2, st))) return rc;
-    StageOut O3{state, ustar, ua, nullptr, 3, 0};
-    if ((rc = launch_stage(C, precision, ub, O3, st))) return rc;
-    StageOut O4{nullptr, ustar, state, nullptr, 4, last ? 0 : 1};
-    if ((rc = launch_stage(C, precision, ua, O4, st))) return rc;
+    StageOut O1{state, ustar, ua, nullptr, 1, 0, nullptr, nullptr};
+    if ((rc = launch_stage(C, precision, state, O1, st, 3, scratch))) return rc;
+    StageOut O2{state, ustar, ub, nullptr, 2, 0, nullptr, nullptr};
+    if ((rc = launch_stage(C, precision, ua, O2, st, 3, scratch))) return rc;
+    StageOut O3{state, ustar, ua, nullptr, 3, 0, nullptr, nullptr};
+    if ((rc = launch_stage(C, precision, ub, O3, st, 3, scratch))) return rc;
+    StageOut O4{nullptr, ustar, state, nullptr, 4, last ? 0 : 1, nullptr, nullptr};
+    if ((rc = launch_stage(C, precision, ua, O4, st, 3, scratch))) return rc;
   }
   return BSSN_OK;
 }
 
-int bssn_constraints(const bssn_params *p, const bssn_grid *grid, const double *state, double *out, void *stream) {
+int bssn_constraints_with_matter(const bssn_params *p, const bssn_grid *grid, const double *state,
+                                 const double *matter, double *out, void *stream) {
   Ctx C;
   bssn_params q = p ? *p : bssn_params{};
   q.kappa = 0.0;  // constraints do not depend on kappa
@@ -546,8 +606,12 @@ int bssn_constraints(const bssn_params *p, const bssn_grid *grid, const double *
   if (rc) return rc;
   if (!state || !out) return fail(BSSN_ERR_INVALID, "null buffer");
   const int64_t blocks = (C.ncell + 255) / 256;
-  hipLaunchKernelGGL(k_constraints, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, C, state, out);
+  hipLaunchKernelGGL(k_constraints, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, C, state, matter, out);
   return check_hip(hipGetLastError(), "constraint kernel launch");
+}
+
+int bssn_constraints(const bssn_params *p, const bssn_grid *grid, const double *state, double *out, void *stream) {
+  return bssn_constraints_with_matter(p, grid, state, nullptr, out, stream);
 }
 
 int64_t bssn_norm_partial_size(const bssn_grid *grid) {
@@ -572,7 +636,7 @@ int bssn_constraint_norms(const bssn_grid *grid, const double *constraints, doub
 
 int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precision, int stage, int part,
                          const double *u_s, const double *u0, double *ustar, double *u_next, int reps, float *ms,
-                         void *stream) {
+                         double *work, void *stream) {
   Ctx C;
   int rc = resolve(p, grid, &C);
   if (rc) return rc;
@@ -588,9 +652,9 @@ int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precis
     (void)hipEventDestroy(e0);
     return rc;
   }
-  StageOut O{stage == 1 ? u_s : u0, ustar, u_next, nullptr, stage, 0};
+  StageOut O{stage == 1 ? u_s : u0, ustar, u_next, nullptr, stage, 0, nullptr, nullptr};
   rc = check_hip(hipEventRecord(e0, st), "hipEventRecord");
-  for (int r = 0; r < reps && !rc; ++r) rc = launch_stage(C, precision, u_s, O, st, part);
+  for (int r = 0; r < reps && !rc; ++r) rc = launch_stage(C, precision, u_s, O, st, part, work);
   if (!rc) rc = check_hip(hipEventRecord(e1, st), "hipEventRecord");
   if (!rc) rc = check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
   float t = 0.0f;

@@ -35,6 +35,9 @@ enum Comp : int {
   NCOMP = 21
 };
 
+// Matter-source components: rho, S_x, S_y, S_z, S_xx S_xy S_xz S_yy S_yz S_zz.
+constexpr int NMATTER = 10;
+
 // sym(i, j): index 0..5 of the symmetric pair (i, j) in xx xy xz yy yz zz order.
 // Closed form (no recursion) so that it folds to a constant in unrolled loops.
 __host__ __device__ constexpr int sym(int i, int j) {
@@ -80,6 +83,12 @@ struct StageOut {
   double *rhs;      // MODE_RHS output
   int mode;         // 0 = write the RHS, 1..4 = RK4 stage
   int reproject;
+  const double *kap;     // kappa term of the A RHS (6 components) or null
+  const double *matter;  // rho, S_i (3), S_ij (6, packed) or null
+  // filled by the launcher from mode (see stage_update()):
+  //   u_next = nbase + wn k;  ustar = sbase + ws k  (sbase null: no update)
+  const double *nbase, *sbase;
+  double wn, ws;
 };
 
 }  // namespace bssn

@@ -284,6 +284,8 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
 #pragma unroll
   for (int c = 0; c < 6; ++c) R[c] -= W * g[c] * conW;
 
+  const T Wf = W > T(1.0e-12) ? W : T(1.0e-12);
+
   // ---- A^ij and the connection term 2 alpha Gamma^i_jk A^jk = 2 alpha gamma^im Gamma_mjk A^jk
 #pragma unroll
   for (int c = 0; c < 6; ++c) A[c] = (T)pv.st.get(ST_A + c);
@@ -303,7 +305,6 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     T s1[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) s1[m] = contract(C1[m], Aup);
-    const T Wf = W > T(1.0e-12) ? W : T(1.0e-12);
     const T six_al_W = T(6) * al / Wf;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -314,12 +315,26 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
         t3 += Aup[sym(i, j)] * dW[j];
         t4 += Aup[sym(i, j)] * da[j];
       }
-      out.put(GAM + i, (double)(rG[i] + T(2) * al * t2 - six_al_W * t3 - T(2) * t4));
+      T src = T(0);  // -16 pi alpha gamma^ij S_j (conformal_connection.py:165-178)
+      if (out.has_matter())
+        src = T(-16.0 * M_PI) * al *
+              (ig[sym(i, 0)] * (T)out.matter(1) + ig[sym(i, 1)] * (T)out.matter(2) + ig[sym(i, 2)] * (T)out.matter(3));
+      out.put(GAM + i, (double)(rG[i] + T(2) * al * t2 - six_al_W * t3 - T(2) * t4 + src));
     }
   }
 
-  // ---- trace K
-  out.put(CK, (double)((T)pv.st.get(ST_RK) - contract(ig, Hs) + al * contract(A, Aup) + al * K * K * T(1.0 / 3.0)));
+  // ---- trace K (+ 4 pi alpha (rho + W^2 gamma^ij S_ij) with matter,
+  //      reference extrinsic_curvature.py:78-87)
+  {
+    T rKv = (T)pv.st.get(ST_RK) - contract(ig, Hs) + al * contract(A, Aup) + al * K * K * T(1.0 / 3.0);
+    if (out.has_matter()) {
+      T S[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) S[c] = (T)out.matter(4 + c);
+      rKv += T(4.0 * M_PI) * al * ((T)out.matter(0) + Wf * Wf * contract(ig, S));
+    }
+    out.put(CK, (double)rKv);
+  }
 
   // ---- everything of the A RHS except transport and dissipation
   T Arest[6];
@@ -327,6 +342,10 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     T Xr[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) Xr[c] = al * R[c] - Hs[c];
+    if (out.has_matter()) {  // - 8 pi alpha W^2 S_ij inside the trace-free part
+#pragma unroll
+      for (int c = 0; c < 6; ++c) Xr[c] -= T(8.0 * M_PI) * al * Wf * Wf * (T)out.matter(4 + c);
+    }
     const T trX = contract(ig, Xr);
     T gA[3][3];  // gamma^kl A_lj
 #pragma unroll
@@ -349,7 +368,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
   for (int c = 0; c < 6; ++c) {
     pv.stage(15 + c);
     if (face) out.put_face(AXX + c, (double)pv.somm(AXX + c, u[AXX + c]));
-    out.put(AXX + c, (double)(Arest[c] + advection(pv, b) + pv.ko()));
+    out.put(AXX + c, (double)(Arest[c] + advection(pv, b) + pv.ko()) + out.extra_A(c));
   }
   out.done();
 }

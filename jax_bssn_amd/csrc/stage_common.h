@@ -37,20 +37,38 @@ BSSN_HD __forceinline__ void project(double (&g)[6], double (&A)[6]) {
 // Receives each component's RHS as soon as it is final and applies the
 // stage update of bssn_rk4_stage (include/bssn_amd.h) right away; the
 // metric and A are held until done() so they can be projected together.
+// The RK4 stage combination as two fused multiply-adds per component
+// (reference time_evolve.py:165-215 with an accumulating ustar):
+//   stage 1: ustar = u0 + dt/6 k,      u_next = u0 + dt/2 k
+//   stage 2: ustar = ustar + dt/3 k,   u_next = u0 + dt/2 k
+//   stage 3: ustar = ustar + dt/3 k,   u_next = u0 + dt k
+//   stage 4:                           u_next = ustar + dt/6 k
+inline void stage_update(StageOut &O, double dt) {
+  const int m = O.mode;
+  O.nbase = (m == 4) ? O.ustar : O.u0;
+  O.sbase = (m == 1) ? O.u0 : ((m == 2 || m == 3) ? O.ustar : nullptr);
+  O.wn = (m == 3) ? dt : ((m == 4) ? dt / 6.0 : 0.5 * dt);
+  O.ws = (m == 1) ? dt / 6.0 : dt / 3.0;
+}
+
 template <class S>
 struct Emitter {
   int64_t n, idx;
-  double w6, w3, w2, w1;  // dt/6, dt/3, dt/2, dt
-  const BSSN_GLOBAL double *u0;
+  double wn, ws;
+  const BSSN_GLOBAL double *nbase, *sbase;
   BSSN_GLOBAL double *ustar, *unext, *rhs;
+  const BSSN_GLOBAL double *kap;
+  const BSSN_GLOBAL double *mat;
   int mode, reproject;
   bool active, face;
   S st;  // holds the metric of the stage update until done() (its own copy of the handle)
   double An[6];
 
   BSSN_HD __forceinline__ Emitter(const Ctx &C, const StageOut &O, int64_t i, bool act, bool on_face, const S &stash)
-      : n(C.ncell), idx(i), w6(C.dt6), w3(C.dt3), w2(C.dt2), w1(C.dt), u0((const BSSN_GLOBAL double *)O.u0), ustar((BSSN_GLOBAL double *)O.ustar),
-        unext((BSSN_GLOBAL double *)O.unext), rhs((BSSN_GLOBAL double *)O.rhs), mode(O.mode),
+      : n(C.ncell), idx(i), wn(O.wn), ws(O.ws), nbase((const BSSN_GLOBAL double *)O.nbase),
+        sbase((const BSSN_GLOBAL double *)O.sbase), ustar((BSSN_GLOBAL double *)O.ustar),
+        unext((BSSN_GLOBAL double *)O.unext), rhs((BSSN_GLOBAL double *)O.rhs),
+        kap((const BSSN_GLOBAL double *)O.kap), mat((const BSSN_GLOBAL double *)O.matter), mode(O.mode),
         reproject(O.reproject), active(act), face(on_face), st(stash) {}
 
   // bulk RHS of component c (ignored on a Sommerfeld face point)
@@ -67,18 +85,8 @@ struct Emitter {
       rhs[at] = k;
       return;
     }
-    double un;
-    if (mode == 1) {
-      const double v0 = u0[at];
-      ustar[at] = v0 + w6 * k;
-      un = v0 + w2 * k;
-    } else if (mode == 4) {
-      un = ustar[at] + w6 * k;
-    } else {
-      const double v0 = u0[at];
-      ustar[at] = ustar[at] + w3 * k;
-      un = v0 + (mode == 2 ? w2 : w1) * k;
-    }
+    const double un = nbase[at] + wn * k;
+    if (sbase) ustar[at] = sbase[at] + ws * k;
     if (c <= GZZ) {
       st.put(ST_GN + c, un);
     } else if (c >= AXX && c <= AZZ) {
@@ -87,6 +95,13 @@ struct Emitter {
       unext[at] = un;
     }
   }
+
+  // matter source k (rho, S_i, S_ij packed) at this point
+  BSSN_HD __forceinline__ bool has_matter() const { return mat != nullptr; }
+  BSSN_HD __forceinline__ double matter(int k) const { return mat[k * n + idx]; }
+
+  // constraint-damping term of A_ij (component c = 0..5), zero without kappa
+  BSSN_HD __forceinline__ double extra_A(int c) const { return kap ? kap[c * n + idx] : 0.0; }
 
   BSSN_HD __forceinline__ void done() {
     if (!active || mode == 0) return;
@@ -206,8 +221,6 @@ inline int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C, char *er
     C->tile_order = ord ? std::atoi(ord) : 0;
   }
   if (p->gauge != 0 && p->gauge != 1) return BSSN_FAIL(BSSN_ERR_INVALID, "gauge must be 0 (harmonic) or 1 (1+log)");
-  if (p->kappa != 0.0)
-    return BSSN_FAIL(BSSN_ERR_UNSUPPORTED, "kappa != 0 (constraint damping) is not implemented yet");
   return BSSN_OK;
 }
 
@@ -241,5 +254,139 @@ inline Slabs make_slabs(const Ctx &C) {
 }
 
 #undef BSSN_FAIL
+
+// ------------------------------------------------- constraint damping
+
+// Derived grid functions of the kappa term (reference
+// bssn/constraints.py:100-122): D[0..5] = gamma^ij, D[6 + 3i + j] =
+// B_i^j = gamma^jk A_ik, D[15 + 3i + j] = B_i^j / max(W, 1e-12).
+constexpr int NDERIVED = 24;
+
+BSSN_HD inline void derived_point(int64_t n, int64_t i, const double *__restrict__ us, double *D) {
+  double g[6], A[6], ig[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    g[c] = us[(GXX + c) * n + i];
+    A[c] = us[(AXX + c) * n + i];
+  }
+  inverse_metric(g, ig);
+  const double W = us[CW * n + i];
+  const double Wf = W > 1.0e-12 ? W : 1.0e-12;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) D[c * n + i] = ig[c];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double B = ig[sym(j, 0)] * A[sym(r, 0)] + ig[sym(j, 1)] * A[sym(r, 1)] + ig[sym(j, 2)] * A[sym(r, 2)];
+      D[(6 + 3 * r + j) * n + i] = B;
+      D[(15 + 3 * r + j) * n + i] = B / Wf;
+    }
+}
+
+// kappa/2 alpha (D_j M_i + D_i M_j) (reference bssn/extrinsic_curvature.py:
+// 204-218) with M_i and the product-rule d_l M_i of bssn/constraints.py:
+// 22-97 and 100-213.  Mixed second derivatives keep the reference
+// orientation: d2X[l, inner] is the outer derivative l of the inner one.
+BSSN_HD inline void kappa_point(const Ctx &C, const double *__restrict__ us, const double *__restrict__ D,
+                                const double *__restrict__ matter, int64_t i, double *kap) {
+  const int z = (int)(i % C.ax[2].n);
+  const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
+  const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
+  const int64_t n = C.ncell;
+  PointProvider<double> ps, pd;
+  ps.setup(C, us, x, y, z);
+  pd.setup(C, D, x, y, z);
+  double ig[6], B[3][3], BW[3][3];
+  for (int c = 0; c < 6; ++c) ig[c] = D[c * n + i];
+  for (int r = 0; r < 3; ++r)
+    for (int j = 0; j < 3; ++j) {
+      B[r][j] = D[(6 + 3 * r + j) * n + i];
+      BW[r][j] = D[(15 + 3 * r + j) * n + i];
+    }
+  const double W = us[CW * n + i], al = us[ALPHA * n + i];
+  const double Wf = W > 1.0e-12 ? W : 1.0e-12;
+  auto second = [&](PointProvider<double> &pv, int inner, int outer) -> double {
+    return inner == outer ? pv.d2(outer) : pv.dmix(inner, outer);
+  };
+  double dW[3], dK[3], dA[6][3], dg[6][3];
+  double M[3] = {0, 0, 0}, dM[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  ps.stage_comp(CW);
+  for (int j = 0; j < 3; ++j) dW[j] = ps.d1(j);
+  // -3 (B/W)_ij d_l d_j W
+  for (int j = 0; j < 3; ++j)
+    for (int l = 0; l < 3; ++l) {
+      const double s2 = second(ps, j, l);
+      for (int r = 0; r < 3; ++r) dM[r][l] -= 3.0 * BW[r][j] * s2;
+    }
+  ps.stage_comp(CK);
+  for (int j = 0; j < 3; ++j) dK[j] = ps.d1(j);
+  for (int r = 0; r < 3; ++r)
+    for (int l = 0; l < 3; ++l) dM[r][l] -= 2.0 / 3.0 * second(ps, r, l);
+  for (int c = 0; c < 6; ++c) {
+    ps.stage_comp(GXX + c);
+    for (int d = 0; d < 3; ++d) dg[c][d] = ps.d1(d);
+  }
+  for (int c = 0; c < 6; ++c) {
+    ps.stage_comp(AXX + c);
+    for (int d = 0; d < 3; ++d) dA[c][d] = ps.d1(d);
+    const double w = (SYM_I[c] == SYM_J[c] ? 1.0 : 2.0) * ig[c];
+    // -1/2 gamma^jk d_l d_i A_jk
+    for (int r = 0; r < 3; ++r)
+      for (int l = 0; l < 3; ++l) dM[r][l] -= 0.5 * w * second(ps, r, l);
+  }
+  for (int r = 0; r < 3; ++r) {
+    for (int j = 0; j < 3; ++j) {
+      pd.stage_comp(6 + 3 * r + j);
+      M[r] += pd.d1(j);                                // d_j B_r^j
+      for (int l = 0; l < 3; ++l) dM[r][l] += second(pd, j, l);
+      pd.stage_comp(15 + 3 * r + j);
+      for (int l = 0; l < 3; ++l) dM[r][l] -= 3.0 * pd.d1(l) * dW[j];
+    }
+  }
+  for (int c = 0; c < 6; ++c) {
+    pd.stage_comp(c);
+    const double mult = SYM_I[c] == SYM_J[c] ? 1.0 : 2.0;
+    for (int l = 0; l < 3; ++l) {
+      const double dinv = pd.d1(l);
+      for (int r = 0; r < 3; ++r) dM[r][l] -= 0.5 * mult * dinv * dA[c][r];
+    }
+  }
+  for (int r = 0; r < 3; ++r) {
+    double t2 = 0.0, t3 = 0.0;
+    for (int c = 0; c < 6; ++c) t2 += (SYM_I[c] == SYM_J[c] ? 1.0 : 2.0) * ig[c] * dA[c][r];
+    for (int j = 0; j < 3; ++j) t3 += B[r][j] * dW[j];
+    M[r] += -0.5 * t2 - 3.0 * t3 / Wf - 2.0 / 3.0 * dK[r];
+  }
+  if (matter) {
+    // M_i - 8 pi S_i and d_l M_i - 8 pi d_l S_i (bssn/constraints.py:216-259)
+    PointProvider<double> pm;
+    pm.setup(C, matter, x, y, z);
+    for (int r = 0; r < 3; ++r) {
+      pm.stage_comp(1 + r);
+      M[r] -= 8.0 * M_PI * matter[(1 + r) * n + i];
+      for (int l = 0; l < 3; ++l) dM[r][l] -= 8.0 * M_PI * pm.d1(l);
+    }
+  }
+  // D_j M_i = d_j M_i - Gamma^k_ij M_k
+  double C1[3][6];
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 6; ++c) C1[a][c] = 0.5 * (dg[sym(a, SYM_I[c])][SYM_J[c]] + dg[sym(a, SYM_J[c])][SYM_I[c]] - dg[c][a]);
+  double DM[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int j = 0; j < 3; ++j) {
+      double gm = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double c2 = ig[sym(k, 0)] * C1[0][sym(r, j)] + ig[sym(k, 1)] * C1[1][sym(r, j)] +
+                          ig[sym(k, 2)] * C1[2][sym(r, j)];
+        gm += c2 * M[k];
+      }
+      DM[r][j] = dM[r][j] - gm;
+    }
+  for (int c = 0; c < 6; ++c) {
+    const int a = SYM_I[c], b = SYM_J[c];
+    kap[c * n + i] = C.kappa / 2.0 * al * (DM[a][b] + DM[b][a]);
+  }
+}
 
 }  // namespace bssn

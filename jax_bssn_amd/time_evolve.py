@@ -37,14 +37,47 @@ def _like(v):
     return v.conformal_factor
 
 
+def _scratch(params, shape, what, device):
+    n = _lib.work_size(params, shape, what)
+    return torch.empty(max(n, 1), dtype=torch.float64, device=device)
+
+
 def rhs_packed(state, params, precision="f64", out=None):
     """RHS of a packed (21, nx, ny, nz) device state."""
     lib = _lib.load()
     shape = tuple(state.shape[1:])
     if out is None:
         out = torch.empty_like(state)
+    work = _scratch(params, shape, 0, state.device)
     _lib.check(lib.bssn_rhs(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(shape)),
-                            precision_code(precision), _ptr(state), _ptr(out), stream_handle()))
+                            precision_code(precision), _ptr(state), _ptr(out), _ptr(work), stream_handle()))
+    return out
+
+
+def pack_matter(energy_density, momentum_density, spatial_stress, device_):
+    """(rho[n], S_i[3,n], S_ij[3,3,n]) -> (10, nx, ny, nz) device tensor."""
+    def t(x):
+        x = x if isinstance(x, torch.Tensor) else torch.as_tensor(__import__("numpy").asarray(x))
+        return x.to(device=device_, dtype=torch.float64)
+    rho = t(energy_density)
+    m = torch.empty((10,) + tuple(rho.shape), dtype=torch.float64, device=device_)
+    m[0].copy_(rho)
+    m[1:4].copy_(t(momentum_density))
+    Sij = t(spatial_stress)
+    for c, (i, j) in enumerate(((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))):
+        m[4 + c].copy_(Sij[i, j])
+    return m
+
+
+def rhs_with_matter_packed(state, matter, params, precision="f64", out=None):
+    lib = _lib.load()
+    shape = tuple(state.shape[1:])
+    if out is None:
+        out = torch.empty_like(state)
+    work = _scratch(params, shape, 0, state.device)
+    _lib.check(lib.bssn_rhs_with_matter(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(shape)),
+                                        precision_code(precision), _ptr(state), _ptr(matter), _ptr(out),
+                                        _ptr(work), stream_handle()))
     return out
 
 
@@ -61,7 +94,7 @@ def evolve_packed(state, params, nsteps, work=None, precision="f64"):
     """rk4_step applied nsteps times to a packed device state, in place."""
     lib = _lib.load()
     if work is None:
-        work = torch.empty((3,) + tuple(state.shape), dtype=torch.float64, device=state.device)
+        work = _scratch(params, tuple(state.shape[1:]), 1, state.device)
     _lib.check(lib.bssn_evolve(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(state.shape[1:])),
                                precision_code(precision), int(nsteps), _ptr(state), _ptr(work),
                                stream_handle()))
@@ -72,6 +105,14 @@ def compute_bssn_rhs(vars, params, precision="f64"):
     """Reference time_evolve.py:41-57 (vacuum RHS + Sommerfeld faces)."""
     state = pack(vars)
     return unpack(rhs_packed(state, params, precision), like=_like(vars))
+
+
+def compute_bssn_rhs_with_matter(vars, params, energy_density, momentum_density, spatial_stress,
+                                 precision="f64"):
+    """Reference time_evolve.py:60-91 (Eulerian matter sources rho, S_i, S_ij)."""
+    state = pack(vars)
+    matter = pack_matter(energy_density, momentum_density, spatial_stress, state.device)
+    return unpack(rhs_with_matter_packed(state, matter, params, precision), like=_like(vars))
 
 
 def enforce_algebraic_constraints(vars):
@@ -108,7 +149,11 @@ class Evolver:
         else:
             self.state = pack(vars_or_state)
         self.shape = tuple(self.state.shape[1:])
-        self.work = torch.empty((3, NCOMP) + self.shape, dtype=torch.float64, device=self.state.device)
+        n = _lib.work_size(params, self.shape, 1)
+        self.workbuf = torch.empty(n, dtype=torch.float64, device=self.state.device)
+        # the three RK4 state buffers, then the stage scratch
+        self.work = self.workbuf[:3 * NCOMP * int(torch.Size(self.shape).numel())].view((3, NCOMP) + self.shape)
+        self.stage_scratch = self.workbuf[3 * NCOMP * int(torch.Size(self.shape).numel()):]
         self._cp = _lib.c_params(params)
         self._cg = _lib.c_grid(self.shape)
         self._prec = precision_code(precision)
@@ -116,14 +161,15 @@ class Evolver:
     def step(self, nsteps=1):
         lib = _lib.load()
         _lib.check(lib.bssn_evolve(ctypes.byref(self._cp), ctypes.byref(self._cg), self._prec, int(nsteps),
-                                   _ptr(self.state), _ptr(self.work), stream_handle()))
+                                   _ptr(self.state), _ptr(self.workbuf), stream_handle()))
         return self
 
     def stage(self, s, u_s, u0, ustar, u_next, reproject=False):
         lib = _lib.load()
         _lib.check(lib.bssn_rk4_stage(ctypes.byref(self._cp), ctypes.byref(self._cg), self._prec, int(s),
                                       _ptr(u_s), _ptr(u0) if u0 is not None else None, _ptr(ustar),
-                                      _ptr(u_next), 1 if reproject else 0, stream_handle()))
+                                      _ptr(u_next), 1 if reproject else 0, _ptr(self.stage_scratch),
+                                      stream_handle()))
 
     def variables(self, like=None):
         return unpack(self.state, like=like)

@@ -343,24 +343,34 @@ def _rhs_W(v, params, D):
     return first + second + third + ko_dissipation(W, params)
 
 
-def _rhs_K(v, params, inv_gamma, D, hess):
-    """extrinsic_curvature.py:31-92 (vacuum)."""
+def _rhs_K(v, params, inv_gamma, D, hess, matter=None):
+    """extrinsic_curvature.py:31-92."""
     alpha, K, A = v.lapse, v.trace_K, v.traceless_K
     first = -np.einsum('ij...,ij...->...', inv_gamma, hess)
     second = alpha * np.einsum('ij...,kl...,ik...,jl...->...', inv_gamma, inv_gamma, A, A)
     third = alpha * K ** 2 / 3.0
     fourth = compute_shift_advection(K, v.shift, params)
-    return first + second + third + fourth + ko_dissipation(K, params)
+    dt_K = first + second + third + fourth
+    if matter is not None:
+        rho, _, S_ij = matter
+        W_floor = np.maximum(v.conformal_factor, W_FLOOR_VALUE)
+        stress_trace = np.einsum('ij...,ij...->...', W_floor ** 2 * inv_gamma, S_ij)
+        dt_K = dt_K + 4.0 * np.pi * alpha * (rho + stress_trace)
+    return dt_K + ko_dissipation(K, params)
 
 
-def _rhs_A(v, params, inv_gamma, D, hess, ricci):
-    """extrinsic_curvature.py:119-224 (vacuum)."""
+def _rhs_A(v, params, inv_gamma, D, hess, ricci, matter=None):
+    """extrinsic_curvature.py:119-224."""
     alpha, K, A = v.lapse, v.trace_K, v.traceless_K
     gamma = v.conformal_metric
     c2 = christoffel_symbols_second_kind(inv_gamma, D.d1('gamma', gamma))
     first = alpha * K * A
     second = -2 * alpha * np.einsum('ik...,kl...,lj...->ij...', A, inv_gamma, A)
-    third = traceless_part(alpha * ricci - hess, gamma, inv_gamma)
+    third = alpha * ricci - hess
+    if matter is not None:
+        W_floor = np.maximum(v.conformal_factor, W_FLOOR_VALUE)
+        third = third - 8.0 * np.pi * alpha * W_floor ** 2 * matter[2]
+    third = traceless_part(third, gamma, inv_gamma)
     gs = _shift_derivatives(v, D)
     fourth = compute_shift_advection(A, v.shift, params)
     fifth = (np.einsum('mi...,mj...->ij...', A, gs) + np.einsum('mj...,mi...->ij...', A, gs))
@@ -369,6 +379,12 @@ def _rhs_A(v, params, inv_gamma, D, hess, ricci):
     dt_A = first + second + third + fourth + fifth + sixth
     if params.kappa != 0.0:
         M, dM = _momentum_constraint_and_derivative(v, inv_gamma, D)
+        if matter is not None:
+            S_i = matter[1]
+            dS = np.stack([np.stack([diff1_physical(S_i[i], d, params) for d in range(3)])
+                           for i in range(3)])   # (i, l) = d_l S_i
+            M = M - 8.0 * np.pi * S_i
+            dM = dM - 8.0 * np.pi * dS
         DjMi = dM - np.einsum('kij...,k...->ij...', c2, M)
         DiMj = np.swapaxes(DjMi, 0, 1)
         dt_A = dt_A + params.kappa / 2 * alpha * (DjMi + DiMj)
@@ -376,8 +392,8 @@ def _rhs_A(v, params, inv_gamma, D, hess, ricci):
     return dt_A + ko_dissipation(A, params)
 
 
-def _rhs_Gamma(v, params, inv_gamma, D):
-    """conformal_connection.py:19-162."""
+def _rhs_Gamma(v, params, inv_gamma, D, matter=None):
+    """conformal_connection.py:19-178."""
     alpha, A, W = v.lapse, v.traceless_K, v.conformal_factor
     W_floor = np.maximum(W, W_FLOOR_VALUE)
     dW = D.d1('W', W)
@@ -405,7 +421,10 @@ def _rhs_Gamma(v, params, inv_gamma, D):
             div_deriv[m] = div_deriv[m] + ddb[n, m, n]
     ninth = (1.0 / 3.0) * np.einsum('im...,m...->i...', inv_gamma, div_deriv)
     dt = first + second + third + fourth + fifth + sixth + seventh + eighth + ninth
-    return dt + ko_dissipation(v.conformal_connection, params)
+    out = dt + ko_dissipation(v.conformal_connection, params)
+    if matter is not None:
+        out = out + -16.0 * np.pi * alpha * np.einsum('ij...,j...->i...', inv_gamma, matter[1])
+    return out
 
 
 def _rhs_lapse(v, params):
@@ -533,6 +552,27 @@ def compute_bssn_rhs(v, params):
     return apply_sommerfeld_boundaries(v, rhs, params)
 
 
+def compute_bssn_rhs_with_matter(v, params, energy_density, momentum_density, spatial_stress):
+    """time_evolve.py:60-91: sources added before the Sommerfeld faces."""
+    v = as_float64(v)
+    matter = (np.asarray(energy_density, np.float64), np.asarray(momentum_density, np.float64),
+              np.asarray(spatial_stress, np.float64))
+    D = _Derivatives(params)
+    inv_gamma = invert_3x3_metric(v.conformal_metric)
+    hess = _w2_lapse_hessian(v, inv_gamma, D)
+    ricci = _w2_ricci(v, inv_gamma, D)
+    rhs = BSSNVariables(
+        conformal_metric=_rhs_gamma(v, params, D),
+        conformal_factor=_rhs_W(v, params, D),
+        traceless_K=_rhs_A(v, params, inv_gamma, D, hess, ricci, matter),
+        trace_K=_rhs_K(v, params, inv_gamma, D, hess, matter),
+        conformal_connection=_rhs_Gamma(v, params, inv_gamma, D, matter),
+        lapse=_rhs_lapse(v, params),
+        shift=_rhs_shift(v, params),
+    )
+    return apply_sommerfeld_boundaries(v, rhs, params)
+
+
 def enforce_unit_determinant_conformal_metric(v):
     """time_evolve.py:94-111."""
     det = determinant_3x3_metric(v.conformal_metric)
@@ -616,6 +656,13 @@ def compute_all_constraints(v, params):
         trace_A=compute_trace_A_violation(v),
         gamma_condition=compute_gamma_constraint(v, params),
     )
+
+
+def compute_all_constraints_with_matter(v, params, energy_density, momentum_density):
+    """constraints.py:415-440."""
+    c = compute_all_constraints(v, params)
+    return c._replace(hamiltonian=c.hamiltonian - 16.0 * np.pi * np.asarray(energy_density),
+                      momentum=c.momentum - 8.0 * np.pi * np.asarray(momentum_density))
 
 
 def compute_constraint_norms(c):

@@ -4,39 +4,63 @@
 // the library never calls it.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "stage_common.h"
 
 using namespace bssn;
 
 namespace {
 
+// Shares the provider's slots the way the kernels' LDS stash does.
+struct SharedStash {
+  RegStash *p;
+  void put(int slot, double x) { p->put(slot, x); }
+  double get(int slot) const { return p->get(slot); }
+};
+
+template <typename T>
 int run(const bssn_params *p, const bssn_grid *g, int mode, const double *us, const double *u0, double *ustar,
-        double *unext, double *rhs, int reproject, char *err, int errlen) {
+        double *unext, double *rhs, int reproject, const double *matter, char *err, int errlen) {
   Ctx C;
   const int rc = resolve(p, g, &C, err, (size_t)errlen);
   if (rc) return rc;
-  StageOut O{mode == 1 ? us : u0, ustar, unext, rhs, mode, reproject};
+  StageOut O{mode == 1 ? us : u0, ustar, unext, rhs, mode, reproject, nullptr, matter};
+  stage_update(O, C.dt);
+  std::vector<double> D, kap;
+  if (C.kappa != 0.0) {
+    D.resize((size_t)NDERIVED * C.ncell);
+    kap.resize((size_t)6 * C.ncell);
+    for (int64_t i = 0; i < C.ncell; ++i) derived_point(C.ncell, i, us, D.data());
+    for (int64_t i = 0; i < C.ncell; ++i) kappa_point(C, us, D.data(), matter, i, kap.data());
+    O.kap = kap.data();
+  }
   for (int x = 0; x < C.ax[0].n; ++x)
     for (int y = 0; y < C.ax[1].n; ++y)
       for (int z = 0; z < C.ax[2].n; ++z) {
-        PointProvider<double> pv;
+        PointProvider<T> pv;
         pv.setup(C, us, x, y, z);
         double u[NCOMP];
         for (int c = 0; c < NCOMP; ++c) u[c] = us[c * C.ncell + pv.idx];
-        Emitter<RegStash> out(C, O, pv.idx, true, pv.face(), pv.st);
-        bssn_rhs_point<double>(pv, C, u, out);
+        Emitter<SharedStash> out(C, O, pv.idx, true, pv.face(), SharedStash{&pv.st});
+        bssn_rhs_point<T>(pv, C, u, out);
       }
   return 0;
 }
 
 }  // namespace
 
-extern "C" int host_rhs(const bssn_params *p, const bssn_grid *g, const double *state, double *rhs, char *err,
-                        int errlen) {
-  return run(p, g, 0, state, nullptr, nullptr, nullptr, rhs, 0, err, errlen);
+extern "C" int host_rhs(const bssn_params *p, const bssn_grid *g, const double *state, const double *matter,
+                        double *rhs, char *err, int errlen) {
+  return run<double>(p, g, 0, state, nullptr, nullptr, nullptr, rhs, 0, matter, err, errlen);
+}
+
+extern "C" int host_rhs_f32(const bssn_params *p, const bssn_grid *g, const double *state, double *rhs, char *err,
+                            int errlen) {
+  return run<float>(p, g, 0, state, nullptr, nullptr, nullptr, rhs, 0, nullptr, err, errlen);
 }
 
 extern "C" int host_stage(const bssn_params *p, const bssn_grid *g, int stage, const double *us, const double *u0,
                           double *ustar, double *unext, int reproject, char *err, int errlen) {
-  return run(p, g, stage, us, u0, ustar, unext, nullptr, reproject, err, errlen);
+  return run<double>(p, g, stage, us, u0, ustar, unext, nullptr, reproject, nullptr, err, errlen);
 }

@@ -54,7 +54,7 @@ def test_version_string(lib):
 
 def _call_rhs(lib, params, shape=(16, 16, 16)):
     return lib.bssn_rhs(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(shape)), 0,
-                        None, None, None)
+                        None, None, None, None)
 
 
 def test_invalid_configurations_raise_value_error(lib):
@@ -81,12 +81,24 @@ def test_stage_argument_validation(lib):
     p = ctypes.byref(_lib.c_params(BSSNParameters(dx=0.1)))
     g = ctypes.byref(_lib.c_grid((8, 8, 8)))
     with pytest.raises(ValueError, match="stage"):
-        _lib.check(lib.bssn_rk4_stage(p, g, 0, 5, None, None, None, None, 0, None))
+        _lib.check(lib.bssn_rk4_stage(p, g, 0, 5, None, None, None, None, 0, None, None))
     buf = ctypes.c_void_p(0x1000)
     with pytest.raises(ValueError, match="alias"):
         _lib.check(lib.bssn_rk4_stage(p, g, 0, 2, buf, ctypes.c_void_p(0x2000), ctypes.c_void_p(0x3000),
-                                      buf, 0, None))
+                                      buf, 0, None, None))
     with pytest.raises(ValueError, match="precision"):
-        _lib.check(lib.bssn_rk4_stage(p, g, 7, 1, buf, buf, ctypes.c_void_p(0x3000),
-                                      ctypes.c_void_p(0x4000), 0, None) if False else
-                   lib.bssn_rhs(p, g, 7, buf, ctypes.c_void_p(0x4000), None))
+        _lib.check(lib.bssn_rhs(p, g, 7, buf, ctypes.c_void_p(0x4000), None, None))
+
+
+def test_work_sizes(lib):
+    from jax_bssn_amd.variables import BSSNParameters
+    g = ctypes.byref(_lib.c_grid((8, 9, 10)))
+    p0 = ctypes.byref(_lib.c_params(BSSNParameters(dx=0.1)))
+    pk = ctypes.byref(_lib.c_params(BSSNParameters(dx=0.1, kappa=0.02)))
+    n = 8 * 9 * 10
+    assert lib.bssn_work_size(p0, g, 0) == 0
+    assert lib.bssn_work_size(p0, g, 1) == 3 * 21 * n
+    assert lib.bssn_work_size(pk, g, 0) == 30 * n
+    assert lib.bssn_work_size(pk, g, 1) == 3 * 21 * n + 30 * n
+    with pytest.raises(ValueError, match="kappa"):
+        _lib.check(lib.bssn_rhs(pk, g, 0, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), None, None))

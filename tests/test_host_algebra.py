@@ -54,11 +54,21 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def host_rhs(host, v, p):
+def pack_matter(rho, S, Sij):
+    m = np.empty((10,) + rho.shape)
+    m[0] = rho
+    m[1:4] = S
+    for c, (i, j) in enumerate(SYM):
+        m[4 + c] = Sij[i, j]
+    return np.ascontiguousarray(m)
+
+
+def host_rhs(host, v, p, matter=None):
     s = pack(v)
     out = np.zeros_like(s)
     err = ctypes.create_string_buffer(512)
-    rc = host.host_rhs(ctypes.byref(_lib.c_params(p)), ctypes.byref(_lib.c_grid(s.shape[1:])), _p(s), _p(out),
+    mp = _p(matter) if matter is not None else None
+    rc = host.host_rhs(ctypes.byref(_lib.c_params(p)), ctypes.byref(_lib.c_grid(s.shape[1:])), _p(s), mp, _p(out),
                        err, 512)
     assert rc == 0, err.value
     return unpack(out)
@@ -84,9 +94,55 @@ def test_host_rhs_matches_oracle(host, case):
     assert max(errs.values()) <= RHS_TOL, errs
 
 
-@pytest.mark.parametrize("case", ["gauge_wave", "sommerfeld_mixed", "mad_periodic"])
+@pytest.mark.parametrize("case", ["gauge_wave", "sommerfeld_mixed", "mad_periodic", "kappa_periodic"])
 def test_host_stage_update_matches_oracle_step(host, case):
     v, p = CASES[case]()
     errs = rel_errors(host_step(host, v, p), OB.rk4_step(v, p))
     print(case, {k: f"{e:.1e}" for k, e in errs.items()})
     assert max(errs.values()) <= STEP_TOL, errs
+
+
+def matter_sources(shape, seed=21):
+    rng = np.random.default_rng(seed)
+    x = [np.arange(n) / n for n in shape]
+    X, Y, Z = np.meshgrid(*x, indexing='ij')
+    tp = 2 * np.pi
+
+    def f(a):
+        k = rng.integers(1, 3, 3)
+        ph = rng.uniform(0, tp, 3)
+        return a * (1.2 + np.sin(tp * k[0] * X + ph[0]) * np.cos(tp * k[1] * Y + ph[1]) * np.sin(tp * k[2] * Z + ph[2]))
+    rho = f(0.02)
+    S = np.stack([f(0.01) for _ in range(3)])
+    Sij = np.zeros((3, 3) + tuple(shape))
+    for i in range(3):
+        for j in range(i, 3):
+            Sij[i, j] = f(0.01)
+            Sij[j, i] = Sij[i, j]
+    return rho, S, Sij
+
+
+@pytest.mark.parametrize("case", ["shift_1plog", "kappa_periodic", "sommerfeld_aligned"])
+def test_host_rhs_with_matter_matches_oracle(host, case):
+    v, p = CASES[case]()
+    rho, S, Sij = matter_sources(v.conformal_factor.shape)
+    ref = OB.compute_bssn_rhs_with_matter(v, p, rho, S, Sij)
+    got = host_rhs(host, v, p, pack_matter(rho, S, Sij))
+    errs = rel_errors(got, ref)
+    print(case, {k: f"{e:.1e}" for k, e in errs.items()})
+    assert max(errs.values()) <= RHS_TOL, errs
+
+
+@pytest.mark.parametrize("case", ["gauge_wave", "shift_1plog", "sommerfeld_all"])
+def test_host_rhs_f32_within_stated_error(host, case):
+    from tests.test_gpu_parity import F32_RHS_TOL
+    v, p = CASES[case]()
+    s = pack(v)
+    out = np.zeros_like(s)
+    err = ctypes.create_string_buffer(512)
+    rc = host.host_rhs_f32(ctypes.byref(_lib.c_params(p)), ctypes.byref(_lib.c_grid(s.shape[1:])), _p(s), _p(out),
+                           err, 512)
+    assert rc == 0, err.value
+    errs = rel_errors(unpack(out), OB.compute_bssn_rhs(v, p))
+    print(case, {k: f"{e:.1e}" for k, e in errs.items()})
+    assert max(errs.values()) <= F32_RHS_TOL, errs

@@ -309,3 +309,28 @@ def test_gauge_wave_convergence_order():
                     + [np.sqrt(np.mean((v.conformal_factor - ex.conformal_factor) ** 2))])
     orders = np.log2(np.array(errs[0]) / np.array(errs[1]))
     assert np.all(orders > 3.8), orders
+
+
+def test_matter_sources_enter_with_reference_coefficients():
+    """Flat space (vacuum RHS = 0): dt K = 4 pi alpha (rho + S), dt A_ij =
+    -8 pi alpha (S_ij - gamma_ij S / 3), dt Gamma^i = -16 pi alpha S_i,
+    H -> H - 16 pi rho, M_i -> M_i - 8 pi S_i."""
+    shape = (8, 9, 10)
+    rng = np.random.default_rng(4)
+    v = _flat(shape)
+    rho = rng.uniform(0.1, 0.2, shape)
+    S = rng.uniform(-0.1, 0.1, (3,) + shape)
+    Sij = rng.uniform(-0.1, 0.1, (3, 3) + shape)
+    Sij = 0.5 * (Sij + np.swapaxes(Sij, 0, 1))
+    p = B.BSSNParameters(dx=0.1, nu=0.0)
+    r = B.compute_bssn_rhs_with_matter(v, p, rho, S, Sij)
+    trS = Sij[0, 0] + Sij[1, 1] + Sij[2, 2]
+    np.testing.assert_allclose(r.trace_K, 4 * np.pi * (rho + trS), atol=1e-14)
+    expA = -8 * np.pi * (Sij - np.eye(3)[:, :, None, None, None] * trS / 3)
+    np.testing.assert_allclose(r.traceless_K, expA, atol=1e-14)
+    np.testing.assert_allclose(r.conformal_connection, -16 * np.pi * S, atol=1e-14)
+    for f in (r.conformal_metric, r.conformal_factor, r.lapse, r.shift):
+        assert np.abs(f).max() < 1e-14
+    c = B.compute_all_constraints_with_matter(v, p, rho, S)
+    np.testing.assert_allclose(c.hamiltonian, -16 * np.pi * rho, atol=1e-13)
+    np.testing.assert_allclose(c.momentum, -8 * np.pi * S, atol=1e-13)
