@@ -63,36 +63,74 @@ def time_steps(J, state, params, steps, warmup, precision):
     return e0.elapsed_time(e1) / 1e3, ev
 
 
-def tile_kernel_roofline(J, ev, precision, reps=3):
-    """Average tile-kernel launch duration per stage (HIP events on the launch
-    stream) and the algorithmic bytes those launches move."""
+KERNEL_NAMES = {("f64", 4): "k_stage_tile<double, 4, 8, 8, false, false>",
+                ("f64", 8): "k_stage_tile<double, 4, 8, 8, false, true>",
+                ("f32rhs", 4): "k_stage_tile<float, 4, 8, 8, false, false>",
+                ("f32rhs", 8): "k_stage_tile<float, 4, 8, 8, false, true>"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_tile_puncture256_traffic.md")
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^3 from the committed counter
+    summary (a line 'traffic_bytes_per_launch <kernel> <n> <bytes>'), else None."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        f = ln.strip("` ").split("\t")
+        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and int(f[2]) == n:
+            return float(f[3])
+    return None
+
+
+def kernel_roofline(J, ev, precision, part, n, reps=3):
+    """HBM roofline of one tile kernel: the algorithmic bytes of the points it
+    updates in each of the four stages over its mean launch duration, timed
+    with HIP events recorded on the launch stream."""
+    import torch
+
     from jax_bssn_amd import _lib
     lib = _lib.load()
     cp, cg = ev._cp, ev._cg
     prec = _lib.PREC_F64 if precision == "f64" else _lib.PREC_F32RHS
-    npts = int(lib.bssn_stage_points(ctypes.byref(cp), ctypes.byref(cg), 1))
+    npts = int(lib.bssn_stage_points(ctypes.byref(cp), ctypes.byref(cg), part))
     u0 = ev.state
     ustar, ua, ub = ev.work[0], ev.work[1], ev.work[2]
     ua.copy_(u0)
+    ustar.copy_(u0)
     total_bytes, total_s, per_stage = 0.0, 0.0, {}
-    st = ctypes.c_void_p(__import__("torch").cuda.current_stream().cuda_stream)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     for s in (1, 2, 3, 4):
         ms = ctypes.c_float(0.0)
         us = u0 if s == 1 else ua
-        out = ub
-        _lib.check(lib.bssn_time_stage_part(ctypes.byref(cp), ctypes.byref(cg), prec, s, 1,
+        _lib.check(lib.bssn_time_stage_part(ctypes.byref(cp), ctypes.byref(cg), prec, s, part,
                                             ctypes.c_void_p(us.data_ptr()), ctypes.c_void_p(u0.data_ptr()),
-                                            ctypes.c_void_p(ustar.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                            ctypes.c_void_p(ustar.data_ptr()), ctypes.c_void_p(ub.data_ptr()),
                                             reps, ctypes.byref(ms), ctypes.c_void_p(ev.stage_scratch.data_ptr()),
                                             st))
         per_stage[s] = ms.value
         total_bytes += BYTES_PER_CELL[s] * npts
         total_s += ms.value / 1e3
     achieved = total_bytes / total_s
+    kernel = KERNEL_NAMES[(precision, part)]
+    traffic = measured_traffic(kernel, n)
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "k_stage_tile", "launch_ms_by_stage": per_stage,
-            "bytes_per_launch_avg": total_bytes / 4, "points_per_launch": npts}
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel,
+            "launch_ms_by_stage": per_stage, "launch_ms_avg": 1e3 * total_s / 4,
+            "algorithmic_bytes_per_launch_avg": total_bytes / 4, "points_per_launch": npts}
+
+
+def copy_calibration(nbytes=1 << 31, reps=3):
+    """A plain device copy of known size (read nbytes, write nbytes), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def cpu_baseline(n=48, steps=1):
@@ -127,6 +165,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -144,13 +184,14 @@ def main():
 
     t64, ev64 = time_steps(J, state.clone(), params, args.steps, args.warmup, "f64")
     rate64 = cells * args.steps / t64
-    roof = tile_kernel_roofline(J, ev64, "f64")
+    roof = kernel_roofline(J, ev64, "f64", 4, n)
+    closure = kernel_roofline(J, ev64, "f64", 8, n)
     del ev64
     extra = {}
     if not args.no_f32:
         t32, ev32 = time_steps(J, state.clone(), params, args.steps, args.warmup, "f32rhs")
         rate32 = cells * args.steps / t32
-        roof32 = tile_kernel_roofline(J, ev32, "f32rhs")
+        roof32 = kernel_roofline(J, ev32, "f32rhs", 4, n)
         del ev32
         extra["f32rhs"] = {"value": rate32, "ms_per_step": 1e3 * t32 / args.steps,
                            "speedup_vs_f64": rate32 / rate64, "tile_kernel_hbm_frac": roof32["frac"]}
@@ -172,9 +213,13 @@ def main():
                                "Sommerfeld faces, KO nu=0.1, dt=dx/4, FP64 RK4",
                    "grid": [n, n, n], "global_batch": 1, "parallelism": "single GPU"},
         "roofline": roof,
+        "closure_tile_kernel": {k: closure[k] for k in ("kernel", "achieved", "frac", "launch_ms_avg",
+                                                         "points_per_launch")},
         "adjoint_forward_ratio": None,
     }
     line.update(extra)
+    if args.copy_calibration:
+        copy_calibration()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -144,16 +144,27 @@ int64_t bssn_norm_partial_size(const bssn_grid *grid);
 int bssn_constraint_norms(const bssn_grid *grid, const double *constraints,
                           double *partial, double *norms, void *stream);
 
-/* Measurement helper (no reference counterpart): launches part 1 (tile
- * kernel) or part 2 (Sommerfeld shell kernel) of RK4 stage `stage` `reps`
- * times between two hipEvents recorded on `stream`, waits for them and
- * stores the mean duration per launch in *ms.  Buffers as bssn_rk4_stage. */
+/* Launch parts of one RK4 stage, for the measurement helpers below:
+ * TILES    both tile kernels (every point the tile grid owns);
+ * SHELL    the Sommerfeld closure-shell kernel (tiles not aligned to faces);
+ * LEAN     the tile kernel without closure rows only (interior tiles when the
+ *          face tiles take closures, otherwise every tile);
+ * CLOSURE  the closure-capable tile kernel only (face tiles). */
+#define BSSN_PART_TILES 1
+#define BSSN_PART_SHELL 2
+#define BSSN_PART_LEAN 4
+#define BSSN_PART_CLOSURE 8
+
+/* Measurement helper (no reference counterpart): launches one part
+ * (BSSN_PART_*) of RK4 stage `stage` `reps` times between two hipEvents
+ * recorded on `stream`, waits for them and stores the mean duration per
+ * launch in *ms.  Buffers as bssn_rk4_stage. */
 int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precision, int stage, int part,
                          const double *u_s, const double *u0, double *ustar, double *u_next, int reps,
                          float *ms, double *work, void *stream);
 
-/* Number of grid points part 1 (tile kernel) or part 2 (shell kernel) of a
- * stage updates (negative status on invalid input). */
+/* Number of grid points one part (BSSN_PART_*) of a stage updates
+ * (negative status on invalid input). */
 int64_t bssn_stage_points(const bssn_params *p, const bssn_grid *grid, int part);
 
 #ifdef __cplusplus

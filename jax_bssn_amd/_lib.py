@@ -17,6 +17,7 @@ BSSN_ERR_HIP = -2
 BSSN_ERR_UNSUPPORTED = -3
 PREC_F64 = 0
 PREC_F32RHS = 1
+PART_TILES, PART_SHELL, PART_LEAN, PART_CLOSURE = 1, 2, 4, 8  # BSSN_PART_*
 
 
 class CParams(ctypes.Structure):

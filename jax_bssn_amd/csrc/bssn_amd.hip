@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -21,6 +22,7 @@
 #include "rhs_point.h"
 #include "stage_common.h"
 #include "stencil_providers.h"
+#include "tiles.h"
 
 // Minimum waves per SIMD the tile kernels are compiled for (caps VGPRs).
 #ifndef BSSN_MIN_WAVES
@@ -49,42 +51,14 @@ __device__ __forceinline__ void load_point(const Ctx &C, const double *__restric
 
 // ------------------------------------------------------------ tile kernel
 
-// tile_class: 0 every tile; 1 only tiles without closure points; 2 only
-// tiles with closure points (the two classes of one Sommerfeld stage run
-// as a lean interior launch plus a closure-capable face launch).
 template <typename T, int TX, int TY, int TZ, bool MAD, bool CL>
 __global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
-    k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, int remap, int tile_class) {
+    k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, TileSet S) {
   using P = TileProvider<T, TX, TY, TZ, MAD, CL>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
-  const int nbx = (C.ax[0].n + TX - 1) / TX;
-  const int nby = (C.ax[1].n + TY - 1) / TY;
-  const int nbz = (C.ax[2].n + TZ - 1) / TZ;
-  const int ntiles = nbx * nby * nbz;
-  const int bid = remap ? xcd_remap((int)blockIdx.x, ntiles) : (int)blockIdx.x;
   int bx, by, bz;
-  if (C.tile_order == 0) {  // z fastest
-    bz = bid % nbz;
-    by = (bid / nbz) % nby;
-    bx = bid / (nbz * nby);
-  } else {  // x fastest
-    bx = bid % nbx;
-    bz = (bid / nbx) % nbz;
-    by = bid / (nbx * nbz);
-  }
-  if (tile_class) {
-    const int o[3] = {bx * TX, by * TY, bz * TZ};
-    const int t[3] = {TX, TY, TZ};
-    bool touch = false;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const Axis &A = C.ax[a];
-      touch |= (A.lbc && o[a] < A.start + 3 && o[a] + t[a] > A.start) ||
-               (A.rbc && o[a] + t[a] > A.end - 3 && o[a] < A.end);
-    }
-    if (touch != (tile_class == 2)) return;  // uniform per block, before any barrier
-  }
+  tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz);
   const int tid = threadIdx.x;
   const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
   const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
@@ -410,6 +384,10 @@ int check_hip(hipError_t e, const char *what) {
 #endif
 constexpr int TX = BSSN_TILE_X, TY = BSSN_TILE_Y, TZ = BSSN_TILE_Z;
 
+// launch parts of a stage (bssn_time_stage_part / bssn_stage_points)
+constexpr int PART_TILES = BSSN_PART_TILES, PART_SHELL = BSSN_PART_SHELL, PART_LEAN = BSSN_PART_LEAN,
+              PART_CLOSURE = BSSN_PART_CLOSURE;
+
 // Closures can be evaluated inside the tiles when every closure axis has its
 // owned range aligned to the tile grid (each face tile starts / ends on the
 // face); otherwise the closure shells go to the shell kernel.
@@ -425,26 +403,28 @@ bool tiles_aligned(const Ctx &C) {
 
 template <typename T, bool MAD, bool CL>
 int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
-  const int nbx = (C.ax[0].n + TX - 1) / TX, nby = (C.ax[1].n + TY - 1) / TY, nbz = (C.ax[2].n + TZ - 1) / TZ;
-  const int ntiles = nbx * nby * nbz;
   int rc = BSSN_OK;
-  if (parts & 1) {
-    if (CL) {
-      // lean kernel on the interior tiles, closure-capable kernel on the face tiles
-      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us, O,
-                         1, 1);
+  // lean kernel: every tile, or with in-tile closures only the interior tiles
+  if (parts & (PART_TILES | PART_LEAN)) {
+    const TileSet S = tile_set(C, CL ? 1 : 0, TX, TY, TZ);
+    if (S.count > 0) {
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
+                         O, S);
       rc = check_hip(hipGetLastError(), "tile kernel launch");
       if (rc) return rc;
-      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us, O,
-                         1, 2);
-    } else {
-      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false>), dim3(ntiles), dim3(TX * TY * TZ), 0, st, C, us,
-                         O, 1, 0);
     }
-    rc = check_hip(hipGetLastError(), "tile kernel launch");
-    if (rc) return rc;
   }
-  if ((parts & 2) && C.any_shell && !CL) {
+  // closure-capable kernel on the face tiles
+  if (CL && (parts & (PART_TILES | PART_CLOSURE))) {
+    const TileSet S = tile_set(C, 2, TX, TY, TZ);
+    if (S.count > 0) {
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
+                         O, S);
+      rc = check_hip(hipGetLastError(), "tile kernel launch");
+      if (rc) return rc;
+    }
+  }
+  if ((parts & PART_SHELL) && C.any_shell && !CL) {
     Slabs S = make_slabs(C);
     const int64_t total = S.first[S.count];
     if (total > 0) {
@@ -478,7 +458,8 @@ int launch_kappa(const Ctx &C, const double *us, const double *matter, double *w
   return check_hip(hipGetLastError(), "kappa kernel launch");
 }
 
-int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &O0, hipStream_t st, int parts = 3,
+int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &O0, hipStream_t st,
+                 int parts = PART_TILES | PART_SHELL,
                  double *work = nullptr) {
   StageOut O = O0;
   stage_update(O, C.dt);
@@ -493,8 +474,13 @@ int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &
   return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
 }
 
-// Points updated by the tile kernel (part 1) or the shell kernel (part 2).
+// Points updated by the kernels of one part (PART_* bits, one at a time).
 int64_t stage_points(const Ctx &C, int part) {
+  const bool cl = C.any_shell && tiles_aligned(C);
+  if (part == PART_LEAN || part == PART_CLOSURE) {
+    if (!cl) return part == PART_LEAN ? stage_points(C, PART_TILES) : 0;
+    return tile_set_points(C, tile_set(C, part == PART_LEAN ? 1 : 2, TX, TY, TZ), TX, TY, TZ);
+  }
   int64_t shell = 0;
   if (C.any_shell && !tiles_aligned(C)) {
     // inclusion-exclusion over the per-axis shells: count the complement
@@ -502,7 +488,11 @@ int64_t stage_points(const Ctx &C, int part) {
     for (int a = 0; a < 3; ++a) inner *= (int64_t)(C.ax[a].n - C.ax[a].shell_lo - C.ax[a].shell_hi);
     shell = C.ncell - inner;
   }
-  return part == 2 ? shell : C.ncell - shell;
+  return part == PART_SHELL ? shell : C.ncell - shell;
+}
+
+bool valid_part(int part) {
+  return part == PART_TILES || part == PART_SHELL || part == PART_LEAN || part == PART_CLOSURE;
 }
 
 int launch_project(int64_t n, const double *in, double *out, hipStream_t st) {
@@ -641,7 +631,7 @@ int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precis
   int rc = resolve(p, grid, &C);
   if (rc) return rc;
   if (stage < 1 || stage > 4) return fail(BSSN_ERR_INVALID, "stage must be 1..4");
-  if (part != 1 && part != 2) return fail(BSSN_ERR_INVALID, "part must be 1 (tile) or 2 (shell)");
+  if (!valid_part(part)) return fail(BSSN_ERR_INVALID, "part must be one of BSSN_PART_*");
   if (reps < 1 || !ms) return fail(BSSN_ERR_INVALID, "reps must be positive and ms non-null");
   if (!u_s || !ustar || !u_next || (stage != 4 && !u0)) return fail(BSSN_ERR_INVALID, "null buffer");
   if (u_next == u_s) return fail(BSSN_ERR_INVALID, "u_next must not alias u_s");
@@ -669,7 +659,7 @@ int64_t bssn_stage_points(const bssn_params *p, const bssn_grid *grid, int part)
   Ctx C;
   int rc = resolve(p, grid, &C);
   if (rc) return rc;
-  if (part != 1 && part != 2) return fail(BSSN_ERR_INVALID, "part must be 1 (tile) or 2 (shell)");
+  if (!valid_part(part)) return fail(BSSN_ERR_INVALID, "part must be one of BSSN_PART_*");
   return stage_points(C, part);
 }
 

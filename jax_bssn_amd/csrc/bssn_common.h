@@ -72,7 +72,6 @@ struct Ctx {
   double xmin[3];
   int gauge, zero_shift;
   int any_shell;    // some axis has a closure shell
-  int tile_order;   // tile traversal: 0 z fastest, 1 x fastest
 };
 
 // Stage-update arguments (see bssn_rk4_stage in include/bssn_amd.h).

@@ -216,10 +216,6 @@ inline int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C, char *er
   C->gauge = p->gauge;
   C->zero_shift = p->zero_shift;
   C->any_shell = any_shell;
-  {
-    const char *ord = std::getenv("BSSN_TILE_ORDER");
-    C->tile_order = ord ? std::atoi(ord) : 0;
-  }
   if (p->gauge != 0 && p->gauge != 1) return BSSN_FAIL(BSSN_ERR_INVALID, "gauge must be 0 (harmonic) or 1 (1+log)");
   return BSSN_OK;
 }

@@ -240,7 +240,11 @@ struct TileProvider {
   static constexpr int SX = LY * LZ, SY = LZ;
   static constexpr int NBOX = LX * LY * LZ;
   static constexpr int NSLOT = (NBOX + NT - 1) / NT;
-  static constexpr int LDS_ELEMS = 2 * NBOX;  // double-buffered tile
+  // Double-buffered when two workgroups (box buffers + stash) still fit the
+  // 160 KB of LDS of a CU; wider boxes (closure or MAD halos) keep one.
+  static constexpr int NBUF =
+      (2 * NBOX * (int)sizeof(T) + STASH_SLOTS * NT * (int)sizeof(double) <= 80 * 1024) ? 2 : 1;
+  static constexpr int LDS_ELEMS = NBUF * NBOX;
 
   BSSN_LDS T *lds;        // two tile buffers
   const BSSN_LDS T *cur;  // buffer of the staged component
@@ -320,10 +324,12 @@ struct TileProvider {
   }
 
   // Stage visit v: store the prefetched tile, one barrier, prefetch v + 1.
-  // Buffer v & 1 was last read in visit v - 2, which every thread finished
-  // before arriving at the barrier of visit v - 1.
+  // Double-buffered, buffer v & 1 was last read in visit v - 2, which every
+  // thread finished before arriving at the barrier of visit v - 1; a single
+  // buffer needs a second barrier before it is overwritten.
   __device__ __forceinline__ void stage(int v) {
-    BSSN_LDS T *buf = lds + (v & 1) * NBOX;
+    BSSN_LDS T *buf = lds + (NBUF == 2 ? (v & 1) * NBOX : 0);
+    if (NBUF == 1 && v > 0) __syncthreads();  // every thread is done with visit v - 1
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;

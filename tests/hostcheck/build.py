@@ -9,7 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 LIB = os.path.join(HERE, "libhostcheck.so")
 SRC = os.path.join(HERE, "host_rhs.cpp")
 DEPS = [SRC] + [os.path.join(ROOT, "jax_bssn_amd", "csrc", f) for f in
-                ("bssn_common.h", "rhs_point.h", "stage_common.h", "stencil_providers.h")] + \
+                ("bssn_common.h", "rhs_point.h", "stage_common.h", "stencil_providers.h", "tiles.h")] + \
        [os.path.join(ROOT, "include", "bssn_amd.h")]
 
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "stage_common.h"
+#include "tiles.h"
 
 using namespace bssn;
 
@@ -64,3 +65,38 @@ extern "C" int host_stage(const bssn_params *p, const bssn_grid *g, int stage, c
                           double *ustar, double *unext, int reproject, char *err, int errlen) {
   return run<double>(p, g, stage, us, u0, ustar, unext, nullptr, reproject, nullptr, err, errlen);
 }
+
+// Tile sets against the brute-force predicate: every tile of kind 1 (2)
+// enumerated exactly once iff it has no (has) closure points, and the point
+// count of the two sets adds up to the grid.  Returns the number of
+// violations (or a negative status).
+extern "C" int host_tile_check(const bssn_params *p, const bssn_grid *g, int tx, int ty, int tz, char *err,
+                               int errlen) {
+  Ctx C;
+  const int rc = resolve(p, g, &C, err, (size_t)errlen);
+  if (rc) return rc;
+  int bad = 0;
+  int64_t pts = 0;
+  for (int kind = 1; kind <= 2; ++kind) {
+    const TileSet S = tile_set(C, kind, tx, ty, tz);
+    std::vector<int> seen((size_t)S.n[0] * S.n[1] * S.n[2], 0);
+    for (int k = 0; k < S.count; ++k) {
+      int bx, by, bz;
+      tile_of(S, k, bx, by, bz);
+      if (bx < 0 || bx >= S.n[0] || by < 0 || by >= S.n[1] || bz < 0 || bz >= S.n[2]) {
+        ++bad;
+        continue;
+      }
+      seen[((size_t)bx * S.n[1] + by) * S.n[2] + bz] += 1;
+    }
+    for (int bx = 0; bx < S.n[0]; ++bx)
+      for (int by = 0; by < S.n[1]; ++by)
+        for (int bz = 0; bz < S.n[2]; ++bz) {
+          const bool want = tile_has_closure(C, bx * tx, by * ty, bz * tz, tx, ty, tz) == (kind == 2);
+          if (seen[((size_t)bx * S.n[1] + by) * S.n[2] + bz] != (want ? 1 : 0)) ++bad;
+        }
+    pts += tile_set_points(C, S, tx, ty, tz);
+  }
+  return bad + (pts != C.ncell ? 1 : 0);
+}
+

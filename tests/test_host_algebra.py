@@ -146,3 +146,26 @@ def test_host_rhs_f32_within_stated_error(host, case):
     errs = rel_errors(unpack(out), OB.compute_bssn_rhs(v, p))
     print(case, {k: f"{e:.1e}" for k, e in errs.items()})
     assert max(errs.values()) <= F32_RHS_TOL, errs
+
+
+@pytest.mark.parametrize("tile", [(4, 8, 8), (8, 8, 8), (2, 8, 16), (8, 8, 4)])
+def test_tile_sets_partition_the_grid(host, tile):
+    """The lean / closure tile launches cover every tile exactly once and
+    split them by the closure predicate, for Sommerfeld on any subset of faces,
+    tiled owned ranges and grids that are not a multiple of the tile."""
+    from jax_bssn_amd.variables import BSSNParameters
+    err = ctypes.create_string_buffer(512)
+    rng = np.random.default_rng(5)
+    configs = [((64, 64, 64), (1,) * 6, {}), ((64, 64, 64), (1, 0, 0, 1, 1, 1), {}),
+               ((40, 24, 36), (0, 0, 1, 1, 0, 0), {}), ((16, 16, 16), (1,) * 6, {}),
+               ((8, 8, 8), (1,) * 6, {}),
+               ((38, 38, 38), (1,) * 6, dict(guard_cells=3, owned_shape=(32, 32, 32),
+                                              owned_faces=(True, False, True, True, False, True)))]
+    for _ in range(6):
+        shape = tuple(int(v) for v in rng.integers(8, 70, 3))
+        configs.append((shape, tuple(int(v) for v in rng.integers(0, 2, 6)), {}))
+    for shape, bc, extra in configs:
+        names = ("xl_bc", "xr_bc", "yl_bc", "yr_bc", "zl_bc", "zr_bc")
+        p = BSSNParameters(dx=0.1, **dict(zip(names, bc)), **extra)
+        rc = host.host_tile_check(ctypes.byref(_lib.c_params(p)), ctypes.byref(_lib.c_grid(shape)), *tile, err, 512)
+        assert rc == 0, (shape, bc, extra, rc, err.value)
