@@ -144,6 +144,30 @@ int64_t bssn_norm_partial_size(const bssn_grid *grid);
 int bssn_constraint_norms(const bssn_grid *grid, const double *constraints,
                           double *partial, double *norms, void *stream);
 
+/* Fixed mesh refinement (vertex-centred, 2:1).  A child patch with parent
+ * active bounds lo..hi (inclusive, per axis) is allocated with
+ * 2*(hi-lo)+1 active vertices plus `ghost` points on each side; the parent
+ * allocation has `parent_ghost` points before its active vertex 0.
+ *
+ * Replaces reference fmr/transfer.py:237-244 fill_tiled_fine_ghosts (and the
+ * frozen serial tests/support/fmr_reference.py:126-143 fill_fine_ghosts):
+ * every child point outside the active box <- separable degree-5 Lagrange
+ * interpolation of the parent on six nodes per axis around the parent
+ * position parent_ghost + lo + (i - ghost)/2.  periodic[3] marks parent
+ * axes that wrap (a ghost-free periodic root); on other axes the six nodes
+ * must lie inside the parent allocation.  All 21 components. */
+int bssn_fmr_prolong(const bssn_grid *parent_grid, int32_t parent_ghost,
+                     const int32_t *periodic, const double *parent,
+                     const bssn_grid *child_grid, const int32_t *lo, int32_t ghost,
+                     double *child, void *stream);
+
+/* Replaces reference fmr/transfer.py:247-252 restrict_tiled_to_coarse
+ * (frozen fmr_reference.py:146-160 restrict_to_coarse): parent vertices
+ * lo..hi <- the coincident active child vertices (every second one). */
+int bssn_fmr_restrict(const bssn_grid *child_grid, const int32_t *lo, int32_t ghost,
+                      const double *child, const bssn_grid *parent_grid,
+                      int32_t parent_ghost, double *parent, void *stream);
+
 /* Launch parts of one RK4 stage, for the measurement helpers below:
  * TILES    both tile kernels (every point the tile grid owns);
  * SHELL    the Sommerfeld closure-shell kernel (tiles not aligned to faces);

@@ -19,6 +19,7 @@
 
 #include "bssn_amd.h"
 #include "bssn_common.h"
+#include "host_errors.h"
 #include "rhs_point.h"
 #include "stage_common.h"
 #include "stencil_providers.h"
@@ -350,10 +351,10 @@ __global__ void __launch_bounds__(64) k_norm_final(int nblocks, int64_t n, const
 using namespace bssn;
 
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
 
-int fail(int code, const char *fmt, ...) {
+int bssn::fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -363,16 +364,15 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+void bssn::set_error(const char *msg) { g_error = msg; }
+
+namespace {
+
 int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
   char err[512] = {0};
   const int rc = bssn::resolve(p, grid, C, err, sizeof err);
   if (rc) g_error = err;
   return rc;
-}
-
-int check_hip(hipError_t e, const char *what) {
-  if (e != hipSuccess) return fail(BSSN_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
-  return BSSN_OK;
 }
 
 // Tile shape and the minimum waves per SIMD the tile kernels are compiled

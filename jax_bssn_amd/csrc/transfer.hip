@@ -1,0 +1,211 @@
+// Companion kernels of the evolution: fixed-mesh-refinement transfers
+// between a parent level and a vertex-centred 2:1 child patch.
+//
+//  * prolongation: the child's guard shell (ghost points of every axis) from
+//    the parent by separable degree-5 Lagrange interpolation on six parent
+//    nodes per axis (reference tests/support/fmr_reference.py:84-143 and
+//    fmr/transfer.py:57-69, 237-244);
+//  * restriction: injection of the coincident active child vertices into
+//    the parent (fmr_reference.py:146-160, fmr/transfer.py:247-252).
+//
+// States are packed (21 components, each C-order with z fastest).  Child
+// allocations are the active vertices plus `ghost` points on each side.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "bssn_amd.h"
+#include "bssn_common.h"
+#include "host_errors.h"
+
+namespace bssn {
+namespace {
+
+struct LevelGeom {
+  int n[3];        // allocation extents
+  int64_t ncell;
+};
+
+struct ProlongGeom {
+  LevelGeom parent, child;
+  int poff;        // parent ghost width: parent active index 0 sits at poff
+  int periodic[3]; // parent axes that wrap
+  int lo[3];       // child origin in parent active indices
+  int g;           // child ghost width
+  int active[3];   // child active extents
+};
+
+// Six Lagrange nodes and weights of child index i along one axis.
+__device__ __forceinline__ void axis_stencil(const ProlongGeom &G, int a, int i, int node[6], double w[6]) {
+  const double t = (double)(G.poff + G.lo[a]) + (double)(i - G.g) * 0.5;
+  const int base = (int)floor(t);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) node[k] = base - 2 + k;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double p = 1.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+      if (m != k) p *= (t - (double)node[m]) / (double)(node[k] - node[m]);
+    w[k] = p;
+  }
+  const int n = G.parent.n[a];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int v = node[k];
+    if (G.periodic[a]) {
+      v %= n;
+      if (v < 0) v += n;
+    }
+    node[k] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fmr_prolong(ProlongGeom G, const double *__restrict__ parent,
+                                                     double *__restrict__ child) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= G.child.ncell) return;
+  const int k = (int)(id % G.child.n[2]);
+  const int j = (int)((id / G.child.n[2]) % G.child.n[1]);
+  const int i = (int)(id / ((int64_t)G.child.n[2] * G.child.n[1]));
+  const bool active = i >= G.g && i < G.g + G.active[0] && j >= G.g && j < G.g + G.active[1] && k >= G.g &&
+                      k < G.g + G.active[2];
+  if (active) return;
+  int nx[6], ny[6], nz[6];
+  double wx[6], wy[6], wz[6];
+  axis_stencil(G, 0, i, nx, wx);
+  axis_stencil(G, 1, j, ny, wy);
+  axis_stencil(G, 2, k, nz, wz);
+  const int64_t sx = (int64_t)G.parent.n[1] * G.parent.n[2], sy = G.parent.n[2];
+  for (int c = 0; c < NCOMP; ++c) {
+    const double *P = parent + c * G.parent.ncell;
+    double v = 0.0;
+    for (int cz = 0; cz < 6; ++cz) {
+      double vy = 0.0;
+      for (int by = 0; by < 6; ++by) {
+        const double *row = P + ny[by] * sy + nz[cz];
+        double vx = 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 6; ++ax) vx += wx[ax] * row[nx[ax] * sx];
+        vy += wy[by] * vx;
+      }
+      v += wz[cz] * vy;
+    }
+    child[c * G.child.ncell + id] = v;
+  }
+}
+
+struct RestrictGeom {
+  LevelGeom parent, child;
+  int poff, lo[3], g, count[3];  // count: coarse vertices covered per axis
+};
+
+__global__ void __launch_bounds__(256) k_fmr_restrict(RestrictGeom G, const double *__restrict__ child,
+                                                      double *__restrict__ parent) {
+  const int64_t total = (int64_t)G.count[0] * G.count[1] * G.count[2];
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= total) return;
+  const int k = (int)(id % G.count[2]);
+  const int j = (int)((id / G.count[2]) % G.count[1]);
+  const int i = (int)(id / ((int64_t)G.count[2] * G.count[1]));
+  const int64_t src = ((int64_t)(G.g + 2 * i) * G.child.n[1] + (G.g + 2 * j)) * G.child.n[2] + (G.g + 2 * k);
+  const int64_t dst = ((int64_t)(G.poff + G.lo[0] + i) * G.parent.n[1] + (G.poff + G.lo[1] + j)) * G.parent.n[2] +
+                      (G.poff + G.lo[2] + k);
+#pragma unroll
+  for (int c = 0; c < NCOMP; ++c) parent[c * G.parent.ncell + dst] = child[c * G.child.ncell + src];
+}
+
+int level_geom(const bssn_grid *g, LevelGeom *L, const char *what) {
+  if (!g) return fail(BSSN_ERR_INVALID, "%s grid is null", what);
+  L->n[0] = g->nx, L->n[1] = g->ny, L->n[2] = g->nz;
+  for (int a = 0; a < 3; ++a)
+    if (L->n[a] < 1) return fail(BSSN_ERR_INVALID, "%s grid extents must be positive", what);
+  L->ncell = (int64_t)L->n[0] * L->n[1] * L->n[2];
+  if (L->ncell >= ((int64_t)1 << 31)) return fail(BSSN_ERR_INVALID, "%s grid has too many points", what);
+  return BSSN_OK;
+}
+
+// Child active extents from its allocation: 2 (hi - lo) + 1 vertices per axis.
+int child_active(const LevelGeom &C, int g, int active[3]) {
+  if (g < 0) return fail(BSSN_ERR_INVALID, "ghost width must be non-negative");
+  for (int a = 0; a < 3; ++a) {
+    active[a] = C.n[a] - 2 * g;
+    if (active[a] < 1 || active[a] % 2 == 0)
+      return fail(BSSN_ERR_INVALID, "child axis %d: %d points minus 2 x %d ghosts is not an odd vertex count", a,
+                  C.n[a], g);
+  }
+  return BSSN_OK;
+}
+
+}  // namespace
+}  // namespace bssn
+
+using namespace bssn;
+
+extern "C" {
+
+int bssn_fmr_prolong(const bssn_grid *parent_grid, int32_t parent_ghost, const int32_t *periodic,
+                     const double *parent, const bssn_grid *child_grid, const int32_t *lo, int32_t ghost,
+                     double *child, void *stream) {
+  ProlongGeom G;
+  int rc = level_geom(parent_grid, &G.parent, "parent");
+  if (!rc) rc = level_geom(child_grid, &G.child, "child");
+  if (rc) return rc;
+  if (!periodic || !lo || !parent || !child) return fail(BSSN_ERR_INVALID, "null argument");
+  if (parent == child) return fail(BSSN_ERR_INVALID, "parent and child must not alias");
+  if (parent_ghost < 0) return fail(BSSN_ERR_INVALID, "parent ghost width must be non-negative");
+  if (parent_ghost > 0 && (periodic[0] || periodic[1] || periodic[2]))
+    return fail(BSSN_ERR_INVALID, "a periodic parent wraps its whole allocation and carries no ghosts");
+  if ((rc = child_active(G.child, ghost, G.active))) return rc;
+  G.poff = parent_ghost;
+  G.g = ghost;
+  for (int a = 0; a < 3; ++a) {
+    G.lo[a] = lo[a];
+    G.periodic[a] = periodic[a] ? 1 : 0;
+    const int pactive = G.parent.n[a] - 2 * parent_ghost;
+    const int hi = lo[a] + (G.active[a] - 1) / 2;
+    if (lo[a] < 0 || hi >= pactive)
+      return fail(BSSN_ERR_INVALID, "child bounds [%d, %d] leave the parent's %d active vertices on axis %d", lo[a],
+                  hi, pactive, a);
+    if (!G.periodic[a]) {
+      // extreme stencils: first and last child allocation points
+      const double t0 = parent_ghost + lo[a] - 0.5 * ghost;
+      const double t1 = parent_ghost + lo[a] + 0.5 * (G.child.n[a] - 1 - ghost);
+      if ((int)std::floor(t0) - 2 < 0 || (int)std::floor(t1) + 3 >= G.parent.n[a])
+        return fail(BSSN_ERR_INVALID, "child prolongation stencil exceeds parent allocation on axis %d", a);
+    }
+  }
+  const int64_t blocks = (G.child.ncell + 255) / 256;
+  hipLaunchKernelGGL(k_fmr_prolong, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, G, parent, child);
+  return check_hip(hipGetLastError(), "prolongation kernel launch");
+}
+
+int bssn_fmr_restrict(const bssn_grid *child_grid, const int32_t *lo, int32_t ghost, const double *child,
+                      const bssn_grid *parent_grid, int32_t parent_ghost, double *parent, void *stream) {
+  RestrictGeom G;
+  int rc = level_geom(parent_grid, &G.parent, "parent");
+  if (!rc) rc = level_geom(child_grid, &G.child, "child");
+  if (rc) return rc;
+  if (!lo || !parent || !child) return fail(BSSN_ERR_INVALID, "null argument");
+  if (parent == child) return fail(BSSN_ERR_INVALID, "parent and child must not alias");
+  if (parent_ghost < 0) return fail(BSSN_ERR_INVALID, "parent ghost width must be non-negative");
+  int active[3];
+  if ((rc = child_active(G.child, ghost, active))) return rc;
+  G.poff = parent_ghost;
+  G.g = ghost;
+  for (int a = 0; a < 3; ++a) {
+    G.lo[a] = lo[a];
+    G.count[a] = (active[a] - 1) / 2 + 1;
+    const int pactive = G.parent.n[a] - 2 * parent_ghost;
+    if (lo[a] < 0 || lo[a] + G.count[a] > pactive)
+      return fail(BSSN_ERR_INVALID, "child bounds leave the parent's %d active vertices on axis %d", pactive, a);
+  }
+  const int64_t total = (int64_t)G.count[0] * G.count[1] * G.count[2];
+  const int64_t blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(k_fmr_restrict, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, G, child, parent);
+  return check_hip(hipGetLastError(), "restriction kernel launch");
+}
+
+}  // extern "C"
