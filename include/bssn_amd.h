@@ -168,6 +168,50 @@ int bssn_fmr_restrict(const bssn_grid *child_grid, const int32_t *lo, int32_t gh
                       const double *child, const bssn_grid *parent_grid,
                       int32_t parent_ghost, double *parent, void *stream);
 
+/* RHS on the box lo..hi-1 of the grid only (every other point of rhs is
+ * left untouched), with the same stencils, closures and Sommerfeld faces as
+ * bssn_rhs on the whole grid.  Serves callers that need the RHS on part of
+ * a support, like the axisymmetric Cartoon step, which keeps only the
+ * centre y plane of its nine-plane support (reference
+ * cartoon/axisymmetry/evolution.py:18-28 evaluates all nine). */
+int bssn_rhs_region(const bssn_params *p, const bssn_grid *grid, int precision,
+                    const int32_t *lo, const int32_t *hi, const double *state,
+                    double *rhs, double *work, void *stream);
+
+/* The RK4 combination of one stage for an RHS k computed elsewhere
+ * (reference time_evolve.py:144-217; weights as bssn_rk4_stage):
+ *   stage 1: ustar = u0 + dt/6 k,  u_next = P(u0 + dt/2 k)
+ *   stage 2: ustar += dt/3 k,      u_next = P(u0 + dt/2 k)
+ *   stage 3: ustar += dt/3 k,      u_next = P(u0 + dt k)
+ *   stage 4:                       u_next = P(ustar + dt/6 k)
+ * Pointwise; u_next may alias u0 in stage 4 only. */
+int bssn_rk4_combine(const bssn_grid *grid, double dt, int stage, const double *k,
+                     const double *u0, double *ustar, double *u_next, int reproject,
+                     void *stream);
+
+/* Axisymmetric Cartoon (rotation about z).  Compact state: extents
+ * (nx, 1, nz), x index 0..3 parity ghosts, 4..nx-1 positive radial
+ * half-cell centres rho = (i - 3.5) dx; support: (nx, 9, nz), y = (j - 4) dx.
+ *
+ * Replaces reference cartoon/axisymmetry/reconstruction.py:131-145
+ * fill_axisymmetric_ghosts: the four ghosts from the positive half by the
+ * parity of a rotation by pi (vectors (-1, -1, +1)), in place. */
+int bssn_cartoon_axi_fill_ghosts(const bssn_grid *compact, double *state, void *stream);
+
+/* Replaces reconstruction.py:255-292 reconstruct_axisymmetric_support:
+ * support <- every component interpolated (six-point Lagrange on the
+ * parity-reflected radial profile with a three-sample 1/r outer buffer,
+ * z_min the z coordinate of index 0) at rho = sqrt(x^2 + y^2) and rotated
+ * by phi about z. */
+int bssn_cartoon_axi_reconstruct(const bssn_grid *compact, double dx, double z_min,
+                                 const double *state, double *support, void *stream);
+
+/* Replaces reconstruction.py:295-304 project_axisymmetric_rhs: compact rhs
+ * <- the centre plane (j = 4) of a support RHS at positive rho, ghosts by
+ * parity.  Only the centre plane's points x >= 4 of support_rhs are read. */
+int bssn_cartoon_axi_project(const bssn_grid *compact, const double *support_rhs,
+                             double *rhs, void *stream);
+
 /* Launch parts of one RK4 stage, for the measurement helpers below:
  * TILES    both tile kernels (every point the tile grid owns);
  * SHELL    the Sommerfeld closure-shell kernel (tiles not aligned to faces);

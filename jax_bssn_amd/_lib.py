@@ -71,6 +71,17 @@ _SIGNATURES = {
     "bssn_fmr_restrict": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.POINTER(CGrid), ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_void_p]),
+    "bssn_rhs_region": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "bssn_rk4_combine": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p]),
+    "bssn_cartoon_axi_fill_ghosts": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_void_p, ctypes.c_void_p]),
+    "bssn_cartoon_axi_reconstruct": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "bssn_cartoon_axi_project": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
     "bssn_time_stage_part": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

@@ -131,6 +131,34 @@ __global__ void __launch_bounds__(256) k_project(int64_t n, const double *__rest
   }
 }
 
+// RK4 stage combination of an RHS computed elsewhere, then projection
+// (weights and bases from stage_update()).
+__global__ void __launch_bounds__(256) k_combine(int64_t n, StageOut O, const double *__restrict__ k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double g[6], A[6];
+  for (int c = 0; c < NCOMP; ++c) {
+    const int64_t at = c * n + i;
+    const double kc = k[at];
+    const double un = O.nbase[at] + O.wn * kc;
+    if (O.sbase) O.ustar[at] = O.sbase[at] + O.ws * kc;
+    if (c >= GXX && c <= GZZ) {
+      g[c - GXX] = un;
+    } else if (c >= AXX && c <= AZZ) {
+      A[c - AXX] = un;
+    } else {
+      O.unext[at] = un;
+    }
+  }
+  project(g, A);
+  if (O.mode == 4 && O.reproject) project(g, A);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    O.unext[(GXX + c) * n + i] = g[c];
+    O.unext[(AXX + c) * n + i] = A[c];
+  }
+}
+
 // ------------------------------------------------------ constraint kernel
 
 // Reference bssn/constraints.py:22-97 (momentum), 271-303 (Hamiltonian),
@@ -474,6 +502,38 @@ int launch_stage(const Ctx &C, int precision, const double *us, const StageOut &
   return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
 }
 
+// RHS on the points of one box [lo, hi) through the closure-capable point
+// kernel (every other point of rhs untouched).
+int launch_region(const Ctx &C, int precision, const double *us, double *rhs, const int *lo, const int *hi,
+                  double *work, hipStream_t st) {
+  StageOut O{nullptr, nullptr, nullptr, rhs, 0, 0, nullptr, nullptr};
+  stage_update(O, C.dt);
+  if (C.kappa != 0.0) {
+    if (!work) return fail(BSSN_ERR_INVALID, "kappa != 0 needs a work buffer of bssn_work_size() doubles");
+    const int rc = launch_kappa(C, us, nullptr, work, st);
+    if (rc) return rc;
+    O.kap = work + (int64_t)NDERIVED * C.ncell;
+  }
+  Slabs S;
+  std::memset(&S, 0, sizeof S);
+  S.count = 1;
+  for (int a = 0; a < 3; ++a) {
+    S.lo[0][a] = lo[a];
+    S.hi[0][a] = hi[a];
+  }
+  S.first[0] = 0;
+  S.first[1] = (int64_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+  if (S.first[1] == 0) return BSSN_OK;
+  const int64_t blocks = (S.first[1] + 255) / 256;
+  if (precision == BSSN_PREC_F64)
+    hipLaunchKernelGGL((k_stage_shell<double>), dim3((unsigned)blocks), dim3(256), 0, st, C, us, O, S);
+  else if (precision == BSSN_PREC_F32RHS)
+    hipLaunchKernelGGL((k_stage_shell<float>), dim3((unsigned)blocks), dim3(256), 0, st, C, us, O, S);
+  else
+    return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
+  return check_hip(hipGetLastError(), "region RHS kernel launch");
+}
+
 // Points updated by the kernels of one part (PART_* bits, one at a time).
 int64_t stage_points(const Ctx &C, int part) {
   const bool cl = C.any_shell && tiles_aligned(C);
@@ -545,6 +605,39 @@ int bssn_enforce_constraints(const bssn_grid *grid, const double *in, double *ou
   const int64_t n = (int64_t)grid->nx * grid->ny * grid->nz;
   if (n <= 0) return fail(BSSN_ERR_INVALID, "empty grid");
   return launch_project(n, in, out, (hipStream_t)stream);
+}
+
+int bssn_rhs_region(const bssn_params *p, const bssn_grid *grid, int precision, const int32_t *lo,
+                    const int32_t *hi, const double *state, double *rhs, double *work, void *stream) {
+  Ctx C;
+  int rc = resolve(p, grid, &C);
+  if (rc) return rc;
+  if (!lo || !hi || !state || !rhs) return fail(BSSN_ERR_INVALID, "null argument");
+  if (state == rhs) return fail(BSSN_ERR_INVALID, "state and rhs must not alias");
+  int l[3], h[3];
+  for (int a = 0; a < 3; ++a) {
+    l[a] = lo[a], h[a] = hi[a];
+    if (l[a] < 0 || h[a] > C.ax[a].n || l[a] > h[a])
+      return fail(BSSN_ERR_INVALID, "region [%d, %d) does not fit axis %d of extent %d", l[a], h[a], a, C.ax[a].n);
+  }
+  return launch_region(C, precision, state, rhs, l, h, work, (hipStream_t)stream);
+}
+
+int bssn_rk4_combine(const bssn_grid *grid, double dt, int stage, const double *k, const double *u0, double *ustar,
+                     double *u_next, int reproject, void *stream) {
+  if (!grid || !k || !ustar || !u_next || (stage != 4 && !u0)) return fail(BSSN_ERR_INVALID, "null argument");
+  if (stage < 1 || stage > 4) return fail(BSSN_ERR_INVALID, "stage must be 1..4");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(BSSN_ERR_INVALID, "dt must be finite and positive");
+  if (u_next == k || (stage != 4 && u_next == u0) || u_next == ustar)
+    return fail(BSSN_ERR_INVALID, "u_next must not alias k, ustar or (before stage 4) u0");
+  const int64_t n = (int64_t)grid->nx * grid->ny * grid->nz;
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || n >= ((int64_t)1 << 31))
+    return fail(BSSN_ERR_INVALID, "grid extents must be positive and below 2^31 points");
+  StageOut O{u0, ustar, u_next, nullptr, stage, reproject, nullptr, nullptr};
+  stage_update(O, dt);
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, O, k);
+  return check_hip(hipGetLastError(), "combination kernel launch");
 }
 
 int bssn_rk4_stage(const bssn_params *p, const bssn_grid *grid, int precision, int stage, const double *u_s,
