@@ -212,6 +212,30 @@ int bssn_cartoon_axi_reconstruct(const bssn_grid *compact, double dx, double z_m
 int bssn_cartoon_axi_project(const bssn_grid *compact, const double *support_rhs,
                              double *rhs, void *stream);
 
+/* Tile layout of a tiled field (reference parallel/layout.py:27-78
+ * TileLayout): counts tiles per axis, each owning tile_shape samples
+ * (+1 on the last tile of an axis with an upper endpoint) padded by
+ * guard_cells on both sides.  On one GPU the tiles are blocks of one buffer,
+ * tile-major: tile (ix, iy, iz) at ((ix*cy + iy)*cz + iz) * ncomp * prod(alloc)
+ * with alloc = tile_shape + 2 guard_cells, components inside a tile back to
+ * back, each C-order with z fastest. */
+typedef struct bssn_tile_layout {
+    int32_t counts[3];
+    int32_t tile_shape[3];
+    int32_t guard_cells;
+    int32_t upper_endpoints[3];
+} bssn_tile_layout;
+
+/* Replaces reference parallel/communication.py:36-147 (make_tiled_halo_op,
+ * refresh_tiled_field): every guard slab of every tile refreshed in an
+ * x -> y -> z sweep, from the neighbouring tile's owned slab (periodic
+ * wrap across the tile grid) or, on a face with boundary code 1 (bc[6] =
+ * xl xr yl yr zl zr), by the exterior rule: 0 linear extrapolation from
+ * the face sample and its inward neighbour, 1 constant, 2 left as it was.
+ * ncomp components per tile, in place. */
+int bssn_tiled_ghost_fill(const bssn_tile_layout *layout, const int32_t *bc, int exterior,
+                          int ncomp, double *field, void *stream);
+
 /* Launch parts of one RK4 stage, for the measurement helpers below:
  * TILES    both tile kernels (every point the tile grid owns);
  * SHELL    the Sommerfeld closure-shell kernel (tiles not aligned to faces);

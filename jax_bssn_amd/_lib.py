@@ -41,6 +41,12 @@ class CGrid(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32)]
 
 
+class CTileLayout(ctypes.Structure):
+    """bssn_tile_layout of include/bssn_amd.h."""
+    _fields_ = [("counts", ctypes.c_int32 * 3), ("tile_shape", ctypes.c_int32 * 3), ("guard_cells", ctypes.c_int32),
+                ("upper_endpoints", ctypes.c_int32 * 3)]
+
+
 _SIGNATURES = {
     "bssn_last_error_string": (ctypes.c_char_p, []),
     "bssn_version": (ctypes.c_char_p, []),
@@ -82,6 +88,8 @@ _SIGNATURES = {
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_cartoon_axi_project": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
+    "bssn_tiled_ghost_fill": (ctypes.c_int, [ctypes.POINTER(CTileLayout), ctypes.POINTER(ctypes.c_int32),
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_time_stage_part": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

@@ -17,7 +17,7 @@ LIB_NAME = "libbssn_amd.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = os.environ.get("BSSN_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bssn_amd.hip", "transfer.hip", "cartoon.hip"]
+SOURCES = ["bssn_amd.hip", "transfer.hip", "cartoon.hip", "halo.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

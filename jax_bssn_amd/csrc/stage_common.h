@@ -189,8 +189,16 @@ inline int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C, char *er
     A.somm_hi = (A.rbc && (!owned_given || count > 1)) ? 1 : 0;
     if (A.reduced && global_somm)
       return BSSN_FAIL(BSSN_ERR_INVALID, "axis %d is reduced and cannot carry a Sommerfeld face", d);
-    if ((A.lbc || A.rbc) && count < 8)
-      return BSSN_FAIL(BSSN_ERR_INVALID, "Sommerfeld closures on axis %d need at least 8 owned points (have %d)", d, count);
+    // the one-sided rows reach seven samples inward from an owned face; in
+    // a tile they may read the neighbour-filled guards (reference
+    // parallel/initialization.py:31-37 counts min(extent, tile + guards))
+    {
+      const int reach = A.lbc && A.rbc ? count : (A.lbc ? n[d] - A.start : (A.rbc ? A.end : 8));
+      if (reach < 8)
+        return BSSN_FAIL(BSSN_ERR_INVALID,
+                         "Sommerfeld closures on axis %d need at least 8 accessible inward samples (have %d)", d,
+                         reach);
+    }
     A.shell_lo = A.lbc ? 3 : 0;
     A.shell_hi = A.rbc ? 3 : 0;
     any_shell |= A.lbc | A.rbc;

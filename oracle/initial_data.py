@@ -1,9 +1,9 @@
 """Closed-form initial data used by the parity tests and the benchmark.
 
 TEST INFRASTRUCTURE ONLY (see oracle/bssn.py).  The product library has its
-own copy of the analytic state builders in ``jax_bssn_amd/initial_data.py``;
-this module restates the reference formulas independently so that the two
-can be compared.
+own analytic state builders in ``jax_bssn_amd/initial_data.py`` (diagonal
+closed forms); this module restates the reference formulas with the general
+tensor algebra so that the two can be compared.
 
 * gauge_wave_analytic_state      tests/initial_data.py:147-205
 * periodic_coordinate_arrays     tests/initial_data.py:62-73
