@@ -121,6 +121,23 @@ def kernel_roofline(J, ev, precision, part, n, reps=3):
             "algorithmic_bytes_per_launch_avg": total_bytes / 4, "points_per_launch": npts}
 
 
+def f32_accuracy(s64, s32, params, steps):
+    """Error of the FP32-RHS state against the FP64 state after the same
+    steps from the same data: per-field max-norm relative difference and the
+    L2 Hamiltonian constraint of both."""
+    from jax_bssn_amd.constraints import NORM_NAMES, constraints_packed, norms_packed
+    groups = {"conformal_metric": (0, 6), "conformal_factor": (6, 7), "traceless_K": (7, 13), "trace_K": (13, 14),
+              "conformal_connection": (14, 17), "lapse": (17, 18), "shift": (18, 21)}
+    rel = {}
+    for name, (a, b) in groups.items():
+        scale = float(s64[a:b].abs().max())
+        rel[name] = float((s32[a:b] - s64[a:b]).abs().max()) / max(scale, 1e-300)
+    h = [dict(zip(NORM_NAMES, norms_packed(constraints_packed(s, params)).cpu().tolist()))["hamiltonian_l2"]
+         for s in (s64, s32)]
+    return {"steps_compared": steps, "max_rel_diff_vs_f64_by_field": rel, "hamiltonian_l2_f64": h[0],
+            "hamiltonian_l2_f32rhs": h[1], "hamiltonian_l2_ratio": h[1] / h[0]}
+
+
 def copy_calibration(nbytes=1 << 31, reps=3):
     """A plain device copy of known size (read nbytes, write nbytes), for
     scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -184,6 +201,7 @@ def main():
 
     t64, ev64 = time_steps(J, state.clone(), params, args.steps, args.warmup, "f64")
     rate64 = cells * args.steps / t64
+    final64 = ev64.state.clone()
     roof = kernel_roofline(J, ev64, "f64", 4, n)
     closure = kernel_roofline(J, ev64, "f64", 8, n)
     del ev64
@@ -191,10 +209,12 @@ def main():
     if not args.no_f32:
         t32, ev32 = time_steps(J, state.clone(), params, args.steps, args.warmup, "f32rhs")
         rate32 = cells * args.steps / t32
+        acc = f32_accuracy(final64, ev32.state, params, args.warmup + args.steps)
         roof32 = kernel_roofline(J, ev32, "f32rhs", 4, n)
         del ev32
         extra["f32rhs"] = {"value": rate32, "ms_per_step": 1e3 * t32 / args.steps,
-                           "speedup_vs_f64": rate32 / rate64, "tile_kernel_hbm_frac": roof32["frac"]}
+                           "speedup_vs_f64": rate32 / rate64, "tile_kernel_hbm_frac": roof32["frac"], **acc}
+    del final64
     torch.cuda.synchronize()
     line = {
         "metric": "BSSN RK4 cell-updates/s at 256^3 FP64 (single puncture)",

@@ -46,12 +46,17 @@ __host__ __device__ inline int wrap_index(int i, int n) {
 template <typename T>
 struct Row8 {
   int o0;
+  int n;  // taps o0 .. o0 + n - 1 are the row's reach; c[k] = 0 for k >= n
   T c[8];
+  // offset of tap k, clamped into the row's reach: a reader that walks all
+  // eight taps never leaves the samples the row is defined on
+  BSSN_HD __forceinline__ int off(int k) const { return o0 + (k < n ? k : n - 1); }
 };
 
 template <typename T>
-BSSN_HD __forceinline__ void row_clear(Row8<T> &r, int o0) {
+BSSN_HD __forceinline__ void row_clear(Row8<T> &r, int o0, int n = 8) {
   r.o0 = o0;
+  r.n = n;
 #pragma unroll
   for (int k = 0; k < 8; ++k) r.c[k] = T(0);
 }
@@ -59,7 +64,7 @@ BSSN_HD __forceinline__ void row_clear(Row8<T> &r, int o0) {
 template <typename T>
 BSSN_HD __forceinline__ void row_set5(Row8<T> &r, int o0, double a0, double a1, double a2, double a3, double a4,
                                       double s) {
-  row_clear(r, o0);
+  row_clear(r, o0, 5);
   r.c[0] = (T)(a0 * s);
   r.c[1] = (T)(a1 * s);
   r.c[2] = (T)(a2 * s);
@@ -86,7 +91,7 @@ BSSN_HD __forceinline__ Row8<T> row_d1(const Axis &A, int p, double inv_dx, bool
     row_set5(r, -2, 1.0 / 12.0, -2.0 / 3.0, 0.0, 2.0 / 3.0, -1.0 / 12.0, inv_dx);
   } else {
     const double q = A.q, w = 1.0 - q;
-    row_clear(r, -3);
+    row_clear(r, -3, 7);
     r.c[0] = (T)(w * (-1.0 / 60.0) * inv_dx);
     r.c[1] = (T)((q * (1.0 / 12.0) + w * (3.0 / 20.0)) * inv_dx);
     r.c[2] = (T)((q * (-2.0 / 3.0) + w * (-3.0 / 4.0)) * inv_dx);
@@ -104,22 +109,22 @@ BSSN_HD __forceinline__ Row8<T> row_d2(const Axis &A, int p, double inv_dx2) {
   const int s = A.start, e = A.end;
   const double h = inv_dx2;
   if (A.rbc && p == e - 1) {
-    row_clear(r, -5);
+    row_clear(r, -5, 6);
     const double v[6] = {-5.0 / 6.0, 61.0 / 12.0, -13.0, 107.0 / 6.0, -77.0 / 6.0, 15.0 / 4.0};
 #pragma unroll
     for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
   } else if (A.rbc && p == e - 2) {
-    row_clear(r, -4);
+    row_clear(r, -4, 6);
     const double v[6] = {1.0 / 12.0, -1.0 / 2.0, 7.0 / 6.0, -1.0 / 3.0, -5.0 / 4.0, 5.0 / 6.0};
 #pragma unroll
     for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
   } else if (A.lbc && p == s + 1) {
-    row_clear(r, -1);
+    row_clear(r, -1, 6);
     const double v[6] = {5.0 / 6.0, -5.0 / 4.0, -1.0 / 3.0, 7.0 / 6.0, -1.0 / 2.0, 1.0 / 12.0};
 #pragma unroll
     for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
   } else if (A.lbc && p == s) {
-    row_clear(r, 0);
+    row_clear(r, 0, 6);
     const double v[6] = {15.0 / 4.0, -77.0 / 6.0, 107.0 / 6.0, -13.0, 61.0 / 12.0, -5.0 / 6.0};
 #pragma unroll
     for (int k = 0; k < 6; ++k) r.c[k] = (T)(v[k] * h);
@@ -127,7 +132,7 @@ BSSN_HD __forceinline__ Row8<T> row_d2(const Axis &A, int p, double inv_dx2) {
     row_set5(r, -2, -1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0, h);
   } else {
     const double q = A.q, w = 1.0 - q;
-    row_clear(r, -3);
+    row_clear(r, -3, 7);
     const double d4[7] = {0.0, -1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0, 0.0};
     const double d6[7] = {1.0 / 90.0, -3.0 / 20.0, 3.0 / 2.0, -49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
 #pragma unroll
@@ -151,14 +156,14 @@ BSSN_HD __forceinline__ Row8<T> row_upwind(const Axis &A, int p, double inv_dx, 
   const double s12 = inv_dx * (1.0 / 12.0), s60 = inv_dx * (1.0 / 60.0);
   if (coeff > 0.0) {
     // D4 forward on -1..3, D6 forward on -2..4
-    row_clear(r, -2);
+    row_clear(r, -2, 7);
     const double d4[7] = {0.0, -3.0, -10.0, 18.0, -6.0, 1.0, 0.0};
     const double d6[7] = {2.0, -24.0, -35.0, 80.0, -30.0, 8.0, -1.0};
 #pragma unroll
     for (int k = 0; k < 7; ++k) r.c[k] = (T)(q * d4[k] * s12 + w * d6[k] * s60);
   } else if (coeff < 0.0) {
     // D4 backward on -3..1, D6 backward on -4..2
-    row_clear(r, -4);
+    row_clear(r, -4, 7);
     const double d4[7] = {0.0, -1.0, 6.0, -18.0, 10.0, 3.0, 0.0};
     const double d6[7] = {1.0, -8.0, 30.0, -80.0, 35.0, 24.0, -2.0};
 #pragma unroll
@@ -191,7 +196,7 @@ BSSN_HD __forceinline__ Row8<T> row_d6(const Axis &A, int p) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) r.c[j] = (T)(kl == 0 ? L[0][j] : (kl == 1 ? L[1][j] : L[2][j]));
   } else {
-    row_clear(r, -3);
+    row_clear(r, -3, 7);
     const double v[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
 #pragma unroll
     for (int k = 0; k < 7; ++k) r.c[k] = (T)v[k];
@@ -252,6 +257,7 @@ struct TileProvider {
   const BSSN_GLOBAL double *src;
   int64_t ncell;
   int goff[NSLOT];
+  int boff;       // tile origin (the FP32 offset of each component)
   T pref[NSLOT];  // register prefetch of the next component's tile
   int c;
   int tid;
@@ -268,10 +274,15 @@ struct TileProvider {
 
   __device__ __forceinline__ static constexpr int stride(int a) { return a == 0 ? SX : (a == 1 ? SY : 1); }
 
+  // In FP32 the box holds each value minus the component's value at the
+  // tile origin (one block-uniform FP64 load): every stencil's weights sum
+  // to zero, so derivatives are unchanged, and FP32 keeps the digits of the
+  // variation instead of those of a large constant background.
   __device__ __forceinline__ void fetch(int comp) {
     const BSSN_GLOBAL double *f = src + comp * ncell;
+    const double base = sizeof(T) == sizeof(double) ? 0.0 : f[boff];
 #pragma unroll
-    for (int k = 0; k < NSLOT; ++k) pref[k] = (T)f[goff[k]];
+    for (int k = 0; k < NSLOT; ++k) pref[k] = (T)(f[goff[k]] - base);
   }
 
   __device__ __forceinline__ void setup(const Ctx &C, const double *u, T *smem, double *stash_mem, int x0, int y0, int z0,
@@ -320,6 +331,7 @@ struct TileProvider {
       int gz = wrap_index(z0 + lz - HZ, C.ax[2].n);
       goff[k] = gx * C.ax[0].stride + gy * C.ax[1].stride + gz;
     }
+    boff = x0 * C.ax[0].stride + y0 * C.ax[1].stride + z0;
     fetch(visit_comp(0));
   }
 
@@ -359,7 +371,7 @@ struct TileProvider {
   __device__ __forceinline__ T apply(int a, const Row8<T> &r) const {
     T s = T(0);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) s += r.c[k] * at(a, r.o0 + k);
+    for (int k = 0; k < 8; ++k) s += r.c[k] * at(a, r.off(k));
     return s;
   }
 
@@ -406,7 +418,7 @@ struct TileProvider {
       for (int j = 0; j < 8; ++j) {
         T inner = T(0);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) inner += ra.c[i] * at2(a, ra.o0 + i, b, rb.o0 + j);
+        for (int i = 0; i < 8; ++i) inner += ra.c[i] * at2(a, ra.off(i), b, rb.off(j));
         s += rb.c[j] * inner;
       }
       return s;
@@ -517,16 +529,19 @@ struct PointProvider {
   BSSN_HD __forceinline__ void stage(int v) { f = src + visit_comp(v) * C->ncell; }
   BSSN_HD __forceinline__ void stage_comp(int comp) { f = src + comp * C->ncell; }
 
+  // FP32 taps are taken relative to the point's own FP64 value (stencil
+  // weights sum to zero), as in the tile provider.
+  BSSN_HD __forceinline__ double centre() const { return sizeof(T) == sizeof(double) ? 0.0 : f[idx]; }
   BSSN_HD __forceinline__ T ld(int a, int oa) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
-    return (T)f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]];
+    return (T)(f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]] - centre());
   }
   BSSN_HD __forceinline__ T ld2(int a, int oa, int b, int ob) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
     q[b] = wrap_index(q[b] + ob, C->ax[b].n);
-    return (T)f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]];
+    return (T)(f[(int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]] - centre());
   }
   BSSN_HD __forceinline__ T apply(int a, const Row8<T> &r) const {
     T s = T(0);
@@ -535,7 +550,7 @@ struct PointProvider {
       if (r.c[k] != T(0)) s += r.c[k] * ld(a, r.o0 + k);
     return s;
   }
-  BSSN_HD __forceinline__ T val() const { return (T)f[idx]; }
+  BSSN_HD __forceinline__ T val() const { return (T)(f[idx] - centre()); }
   BSSN_HD __forceinline__ T d1(int a) const {
     if (C->ax[a].reduced) return T(0);
     return apply(a, row_d1<T>(C->ax[a], p[a], C->inv_dx, false));

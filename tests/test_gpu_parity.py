@@ -175,3 +175,23 @@ def test_rhs_with_matter_matches_oracle(bssn, case):
     for name in ("hamiltonian", "momentum"):
         a, b = np.asarray(getattr(cgot, name)), np.asarray(getattr(cref, name))
         assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max(), name
+
+
+def test_f32rhs_puncture_face_tiles_stay_finite(bssn):
+    """Regression: the face-tile kernel's mixed derivatives once read
+    zero-weight taps outside the LDS box; harmless in FP64, NaN in FP32.
+    The 128^3 puncture (tile-aligned Sommerfeld faces) exercises every face
+    tile; the FP32-RHS RHS must be finite and within F32_RHS_TOL."""
+    import bench
+    from jax_bssn_amd.time_evolve import rhs_packed
+    state, dx, x0 = bench.puncture_state(128)
+    p = bench.puncture_params(bssn.BSSNParameters, dx, x0)
+    r32 = rhs_packed(state, p, "f32rhs")
+    r64 = rhs_packed(state, p, "f64")
+    assert bool(torch.isfinite(r32).all())
+    scale = r64.abs().reshape(21, -1).max(1).values
+    err = (r32 - r64).abs().reshape(21, -1).max(1).values
+    live = scale > 1e-12
+    worst = float((err[live] / scale[live]).max())
+    print("f32rhs puncture 128^3 rhs worst", f"{worst:.1e}", flush=True)
+    assert worst <= F32_RHS_TOL
