@@ -236,6 +236,31 @@ typedef struct bssn_tile_layout {
 int bssn_tiled_ghost_fill(const bssn_tile_layout *layout, const int32_t *bc, int exterior,
                           int ncomp, double *field, void *stream);
 
+/* Batched evolution (no reference counterpart: the reference evolves one
+ * grid per call; this advances many independent periodic evolutions of
+ * uneven sizes with one persistent launch per RK4 stage).  Each instance
+ * has its own parameters (periodic faces, mad_q = 1, kappa = 0; others
+ * return BSSN_ERR_UNSUPPORTED) and packed state.
+ *
+ * bssn_batch_prepare uploads the instance descriptors into work (doubles:
+ * bssn_batch_work_size) and fills *plan; it waits for the upload.
+ * tile_variant selects the tile shape (0: 4x8x8, 1: 8x8x4, 2: 8x4x8),
+ * grid_blocks the number of persistent workgroups per stage launch.
+ * bssn_batch_step then applies rk4_step (reference time_evolve.py:144-217)
+ * once to every instance, in place: one projection launch and four stage
+ * launches, nothing on the host in between (capturable in a HIP graph).
+ * The states and work must stay where they were when the plan was made. */
+typedef struct bssn_batch_plan {
+    int32_t ninst, total_tiles, precision, tile_variant, grid_blocks, reserved;
+    int64_t total_cells, desc_offset, prefix_offset, proj_offset;
+} bssn_batch_plan;
+
+int64_t bssn_batch_work_size(const bssn_grid *grids, int n);
+int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double *const *states,
+                       int n, int precision, int tile_variant, int grid_blocks, double *work,
+                       bssn_batch_plan *plan, void *stream);
+int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream);
+
 /* Launch parts of one RK4 stage, for the measurement helpers below:
  * TILES    both tile kernels (every point the tile grid owns);
  * SHELL    the Sommerfeld closure-shell kernel (tiles not aligned to faces);

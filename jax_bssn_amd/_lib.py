@@ -47,6 +47,14 @@ class CTileLayout(ctypes.Structure):
                 ("upper_endpoints", ctypes.c_int32 * 3)]
 
 
+class CBatchPlan(ctypes.Structure):
+    """bssn_batch_plan of include/bssn_amd.h."""
+    _fields_ = [("ninst", ctypes.c_int32), ("total_tiles", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("tile_variant", ctypes.c_int32), ("grid_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("total_cells", ctypes.c_int64), ("desc_offset", ctypes.c_int64), ("prefix_offset", ctypes.c_int64),
+                ("proj_offset", ctypes.c_int64)]
+
+
 _SIGNATURES = {
     "bssn_last_error_string": (ctypes.c_char_p, []),
     "bssn_version": (ctypes.c_char_p, []),
@@ -90,6 +98,12 @@ _SIGNATURES = {
                                                 ctypes.c_void_p]),
     "bssn_tiled_ghost_fill": (ctypes.c_int, [ctypes.POINTER(CTileLayout), ctypes.POINTER(ctypes.c_int32),
                                              ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "bssn_batch_work_size": (ctypes.c_int64, [ctypes.POINTER(CGrid), ctypes.c_int]),
+    "bssn_batch_prepare": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid),
+                                          ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(CBatchPlan),
+                                          ctypes.c_void_p]),
+    "bssn_batch_step": (ctypes.c_int, [ctypes.POINTER(CBatchPlan), ctypes.c_void_p, ctypes.c_void_p]),
     "bssn_time_stage_part": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(CGrid), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

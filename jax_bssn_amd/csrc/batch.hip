@@ -1,0 +1,263 @@
+// Batched evolution: many independent periodic evolutions of uneven grid
+// sizes advanced by one persistent launch per RK4 stage.
+//
+// A prepared batch lives in a caller-owned work buffer: per instance its RK4
+// buffers and four stage descriptors (resolved parameters, stage buffers and
+// weights, tile counts, first tile), plus prefix sums of tiles and points.
+// Each stage is one launch of a fixed grid of workgroups that walk the
+// global tile list (tile t of instance i at prefix[i] + local index) and run
+// the same per-point RHS and in-register stage update as the single-grid
+// tile kernel.  A step is a batched projection plus the four stage launches,
+// with no host work in between, so it can be captured in a HIP graph.
+//
+// Restricted to what the batched configuration needs: periodic faces,
+// mad_q = 1 and kappa = 0 (every other configuration runs through the
+// single-grid entry points).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "bssn_amd.h"
+#include "bssn_common.h"
+#include "host_errors.h"
+#include "rhs_point.h"
+#include "stage_common.h"
+#include "stencil_providers.h"
+
+#ifndef BSSN_MIN_WAVES
+#define BSSN_MIN_WAVES 2
+#endif
+
+namespace bssn {
+namespace {
+
+struct BatchDesc {
+  Ctx C;
+  StageOut O;
+  const double *us;
+  int nb[3];        // tiles per axis
+  int tile_begin;   // first global tile of the instance
+};
+
+struct ProjDesc {
+  double *state;
+  int64_t ncell;
+  int64_t cell_begin;
+};
+
+constexpr int BATCH_VARIANTS = 3;
+__host__ __device__ constexpr int variant_tile(int v, int a) {
+  // 0: 4 x 8 x 8, 1: 8 x 8 x 4, 2: 8 x 4 x 8 (256 threads each)
+  return v == 0 ? (a == 0 ? 4 : 8) : (v == 1 ? (a == 2 ? 4 : 8) : (a == 1 ? 4 : 8));
+}
+
+template <typename T, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
+    k_batch_tile(const BatchDesc *__restrict__ D, const int *__restrict__ prefix, int ninst, int total) {
+  using P = TileProvider<T, TX, TY, TZ, false, false>;
+  __shared__ T smem[P::LDS_ELEMS];
+  __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
+  const int tid = threadIdx.x;
+  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {  // last instance whose first tile is <= t (block-uniform)
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchDesc &d = D[lo];
+    const Ctx &C = d.C;
+    const int k = t - d.tile_begin;
+    const int bz = k % d.nb[2], by = (k / d.nb[2]) % d.nb[1], bx = k / (d.nb[2] * d.nb[1]);
+    const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
+    const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
+    const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;
+    const int yc = y < C.ax[1].n ? y : C.ax[1].n - 1;
+    const int zc = z < C.ax[2].n ? z : C.ax[2].n - 1;
+    const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
+    P pv;
+    pv.setup(C, d.us, smem, stash, bx * TX, by * TY, bz * TZ, tx, ty, tz);
+    double u[NCOMP];
+    {
+      const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)d.us;
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) u[c] = g[c * C.ncell + idx];
+    }
+    Emitter<LdsStash<TX * TY * TZ>> out(C, d.O, idx, inside, false, pv.st);
+    bssn_rhs_point<T>(pv, C, u, out);
+    __syncthreads();  // the next tile restages the LDS box
+  }
+}
+
+__global__ void __launch_bounds__(256) k_batch_project(const ProjDesc *__restrict__ D, int ninst, int64_t total) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (D[mid].cell_begin <= gid) lo = mid;
+    else hi = mid - 1;
+  }
+  const ProjDesc d = D[lo];
+  const int64_t n = d.ncell, i = gid - d.cell_begin;
+  double g[6], A[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    g[c] = d.state[(GXX + c) * n + i];
+    A[c] = d.state[(AXX + c) * n + i];
+  }
+  project(g, A);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    d.state[(GXX + c) * n + i] = g[c];
+    d.state[(AXX + c) * n + i] = A[c];
+  }
+}
+
+// Work-buffer layout, in doubles, of a batch of n instances.
+struct BatchLayout {
+  int64_t desc, prefix, proj, bufs, total;
+};
+
+inline int64_t dwords(int64_t bytes) { return (bytes + 7) / 8; }
+
+BatchLayout batch_layout(const bssn_grid *grids, int n) {
+  BatchLayout L;
+  L.desc = 0;
+  L.prefix = L.desc + dwords((int64_t)4 * n * sizeof(BatchDesc));
+  L.proj = L.prefix + dwords((int64_t)(n + 1) * sizeof(int));
+  L.bufs = L.proj + dwords((int64_t)n * sizeof(ProjDesc));
+  int64_t cells = 0;
+  for (int i = 0; i < n; ++i) cells += (int64_t)grids[i].nx * grids[i].ny * grids[i].nz;
+  L.total = L.bufs + 3 * NCOMP * cells;
+  return L;
+}
+
+template <typename T>
+int launch_batch_t(int variant, const BatchDesc *D, const int *prefix, int ninst, int total, int blocks,
+                   hipStream_t st) {
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((k_batch_tile<T, 4, 8, 8>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 8, 4>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      break;
+    default:
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 4, 8>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      break;
+  }
+  return check_hip(hipGetLastError(), "batched stage kernel launch");
+}
+
+}  // namespace
+}  // namespace bssn
+
+using namespace bssn;
+
+extern "C" {
+
+int64_t bssn_batch_work_size(const bssn_grid *grids, int n) {
+  if (!grids || n < 1) return fail(BSSN_ERR_INVALID, "a batch needs at least one grid");
+  return batch_layout(grids, n).total;
+}
+
+int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double *const *states, int n,
+                       int precision, int tile_variant, int grid_blocks, double *work, bssn_batch_plan *plan,
+                       void *stream) {
+  if (!params || !grids || !states || !work || !plan || n < 1) return fail(BSSN_ERR_INVALID, "null argument");
+  if (precision != BSSN_PREC_F64 && precision != BSSN_PREC_F32RHS)
+    return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
+  if (tile_variant < 0 || tile_variant >= BATCH_VARIANTS)
+    return fail(BSSN_ERR_INVALID, "tile_variant must be 0..%d", BATCH_VARIANTS - 1);
+  if (grid_blocks < 1) return fail(BSSN_ERR_INVALID, "grid_blocks must be positive");
+  const BatchLayout L = batch_layout(grids, n);
+  std::vector<BatchDesc> descs((size_t)4 * n);
+  std::vector<int> prefix((size_t)n + 1);
+  std::vector<ProjDesc> proj((size_t)n);
+  int64_t tiles = 0, cells = 0, buf_off = L.bufs;
+  for (int i = 0; i < n; ++i) {
+    Ctx C;
+    char err[512] = {0};
+    int rc = resolve(&params[i], &grids[i], &C, err, sizeof err);
+    if (rc) return fail(rc, "instance %d: %s", i, err);
+    if (!states[i]) return fail(BSSN_ERR_INVALID, "instance %d: null state", i);
+    if (C.any_shell || C.kappa != 0.0 || C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0)
+      return fail(BSSN_ERR_UNSUPPORTED,
+                  "instance %d: the batched launch takes periodic faces, mad_q = 1 and kappa = 0", i);
+    int nb[3];
+    for (int a = 0; a < 3; ++a) nb[a] = (C.ax[a].n + variant_tile(tile_variant, a) - 1) / variant_tile(tile_variant, a);
+    const int64_t nt = (int64_t)nb[0] * nb[1] * nb[2];
+    if (tiles + nt >= ((int64_t)1 << 31)) return fail(BSSN_ERR_INVALID, "batch has too many tiles");
+    prefix[i] = (int)tiles;
+    double *u = states[i];
+    double *ustar = work + buf_off, *ua = ustar + NCOMP * C.ncell, *ub = ua + NCOMP * C.ncell;
+    buf_off += 3 * NCOMP * C.ncell;
+    const double *ins[4] = {u, ua, ub, ua};
+    double *outs[4] = {ua, ub, ua, u};
+    for (int s = 0; s < 4; ++s) {
+      BatchDesc &d = descs[(size_t)s * n + i];
+      std::memset(&d, 0, sizeof d);
+      d.C = C;
+      d.O = StageOut{s == 3 ? nullptr : u, ustar, outs[s], nullptr, s + 1, 0, nullptr, nullptr};
+      stage_update(d.O, C.dt);
+      d.us = ins[s];
+      for (int a = 0; a < 3; ++a) d.nb[a] = nb[a];
+      d.tile_begin = (int)tiles;
+    }
+    proj[i] = ProjDesc{u, C.ncell, cells};
+    tiles += nt;
+    cells += C.ncell;
+  }
+  prefix[n] = (int)tiles;
+  hipStream_t st = (hipStream_t)stream;
+  char *base = (char *)work;
+  int rc = check_hip(hipMemcpyAsync(base + L.desc * 8, descs.data(), descs.size() * sizeof(BatchDesc),
+                                    hipMemcpyHostToDevice, st), "batch descriptor upload");
+  if (!rc) rc = check_hip(hipMemcpyAsync(base + L.prefix * 8, prefix.data(), prefix.size() * sizeof(int),
+                                         hipMemcpyHostToDevice, st), "batch prefix upload");
+  if (!rc) rc = check_hip(hipMemcpyAsync(base + L.proj * 8, proj.data(), proj.size() * sizeof(ProjDesc),
+                                         hipMemcpyHostToDevice, st), "batch projection upload");
+  // the host vectors go out of scope on return: wait for the copies
+  if (!rc) rc = check_hip(hipStreamSynchronize(st), "batch upload");
+  if (rc) return rc;
+  plan->ninst = n;
+  plan->total_tiles = (int32_t)tiles;
+  plan->precision = precision;
+  plan->tile_variant = tile_variant;
+  plan->grid_blocks = grid_blocks;
+  plan->total_cells = cells;
+  plan->desc_offset = L.desc;
+  plan->prefix_offset = L.prefix;
+  plan->proj_offset = L.proj;
+  return BSSN_OK;
+}
+
+int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream) {
+  if (!plan || !work) return fail(BSSN_ERR_INVALID, "null argument");
+  if (plan->ninst < 1 || plan->total_tiles < 1 || plan->grid_blocks < 1 || plan->tile_variant < 0 ||
+      plan->tile_variant >= BATCH_VARIANTS)
+    return fail(BSSN_ERR_INVALID, "batch plan is not prepared");
+  hipStream_t st = (hipStream_t)stream;
+  const BatchDesc *D = (const BatchDesc *)(work + plan->desc_offset);
+  const int *prefix = (const int *)(work + plan->prefix_offset);
+  const ProjDesc *PD = (const ProjDesc *)(work + plan->proj_offset);
+  const int64_t pblocks = (plan->total_cells + 255) / 256;
+  hipLaunchKernelGGL(k_batch_project, dim3((unsigned)pblocks), dim3(256), 0, st, PD, plan->ninst, plan->total_cells);
+  int rc = check_hip(hipGetLastError(), "batched projection launch");
+  const int blocks = plan->grid_blocks < plan->total_tiles ? plan->grid_blocks : plan->total_tiles;
+  for (int s = 0; s < 4 && !rc; ++s) {
+    const BatchDesc *Ds = D + (size_t)s * plan->ninst;
+    rc = plan->precision == BSSN_PREC_F64
+             ? launch_batch_t<double>(plan->tile_variant, Ds, prefix, plan->ninst, plan->total_tiles, blocks, st)
+             : launch_batch_t<float>(plan->tile_variant, Ds, prefix, plan->ninst, plan->total_tiles, blocks, st);
+  }
+  return rc;
+}
+
+}  // extern "C"

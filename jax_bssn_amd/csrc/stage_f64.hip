@@ -1,0 +1,14 @@
+// The double instantiation of the fused stage kernels (stage_kernels.h).
+#include "stage_kernels.h"
+
+namespace bssn {
+
+int launch_stage_f64(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
+  return launch_stage_p<double>(C, us, O, st, parts);
+}
+
+int launch_slabs_f64(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st) {
+  return launch_slabs_t<double>(C, us, O, S, st);
+}
+
+}  // namespace bssn

@@ -1,0 +1,185 @@
+// The fused RK4 stage kernels (tile kernels and closure-shell kernel) and
+// their launchers, templated on the RHS arithmetic type.  Each precision is
+// instantiated in a translation unit of its own (stage_f64.hip,
+// stage_f32.hip) so that the two compile in parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "bssn_amd.h"
+#include "bssn_common.h"
+#include "host_errors.h"
+#include "rhs_point.h"
+#include "stage_common.h"
+#include "stencil_providers.h"
+#include "tiles.h"
+
+// Minimum waves per SIMD the tile kernels are compiled for (caps VGPRs).
+#ifndef BSSN_MIN_WAVES
+#define BSSN_MIN_WAVES 2
+#endif
+
+namespace bssn {
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so tiles sharing halos share an L2.
+__device__ __forceinline__ int xcd_remap(int i, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X;
+  const int x = i % X, l = i / X;
+  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
+__device__ __forceinline__ void load_point(const Ctx &C, const double *__restrict__ u, int64_t idx,
+                                           double (&v)[NCOMP]) {
+  const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)u;
+#pragma unroll
+  for (int c = 0; c < NCOMP; ++c) v[c] = g[c * C.ncell + idx];
+}
+
+// ------------------------------------------------------------ tile kernel
+
+template <typename T, int TX, int TY, int TZ, bool MAD, bool CL>
+__global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
+    k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, TileSet S) {
+  using P = TileProvider<T, TX, TY, TZ, MAD, CL>;
+  __shared__ T smem[P::LDS_ELEMS];
+  __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
+  int bx, by, bz;
+  tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz);
+  const int tid = threadIdx.x;
+  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
+  const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
+  const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;
+  const int yc = y < C.ax[1].n ? y : C.ax[1].n - 1;
+  const int zc = z < C.ax[2].n ? z : C.ax[2].n - 1;
+  const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
+
+  P pv;
+  pv.setup(C, us, smem, stash, bx * TX, by * TY, bz * TZ, tx, ty, tz);
+  double u[NCOMP];
+  load_point(C, us, idx, u);
+  // without in-tile closures the shell points belong to the shell kernel
+  const bool mine = inside && (CL || !(C.any_shell && in_shell(C, x, y, z)));
+  Emitter<LdsStash<TX * TY * TZ>> out(C, O, idx, mine, pv.face(), pv.st);
+  bssn_rhs_point<T>(pv, C, u, out);
+}
+
+// ----------------------------------------------------------- shell kernel
+
+__device__ __forceinline__ bool in_slab(const Slabs &S, int s, int x, int y, int z) {
+  return x >= S.lo[s][0] && x < S.hi[s][0] && y >= S.lo[s][1] && y < S.hi[s][1] && z >= S.lo[s][2] &&
+         z < S.hi[s][2];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_stage_shell(Ctx C, const double *__restrict__ us, StageOut O, Slabs S) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= S.first[S.count]) return;
+  int s = 0;
+  while (s + 1 < S.count && gid >= S.first[s + 1]) ++s;
+  const int64_t l = gid - S.first[s];
+  const int ez = S.hi[s][2] - S.lo[s][2], ey = S.hi[s][1] - S.lo[s][1];
+  const int z = S.lo[s][2] + (int)(l % ez);
+  const int y = S.lo[s][1] + (int)((l / ez) % ey);
+  const int x = S.lo[s][0] + (int)(l / ((int64_t)ez * ey));
+  for (int t = 0; t < s; ++t)
+    if (in_slab(S, t, x, y, z)) return;
+
+  PointProvider<T> pv;
+  pv.setup(C, us, x, y, z);
+  double u[NCOMP];
+  load_point(C, us, pv.idx, u);
+  Emitter<RegStash> out(C, O, pv.idx, true, pv.face(), pv.st);
+  bssn_rhs_point<T>(pv, C, u, out);
+}
+
+// Tile shape and the minimum waves per SIMD the tile kernels are compiled
+// for (overridable at build time to produce tuning variants).
+#ifndef BSSN_TILE_X
+#define BSSN_TILE_X 4
+#define BSSN_TILE_Y 8
+#define BSSN_TILE_Z 8
+#endif
+constexpr int TX = BSSN_TILE_X, TY = BSSN_TILE_Y, TZ = BSSN_TILE_Z;
+
+// launch parts of a stage (bssn_time_stage_part / bssn_stage_points)
+constexpr int PART_TILES = BSSN_PART_TILES, PART_SHELL = BSSN_PART_SHELL, PART_LEAN = BSSN_PART_LEAN,
+              PART_CLOSURE = BSSN_PART_CLOSURE;
+
+// Closures can be evaluated inside the tiles when every closure axis has its
+// owned range aligned to the tile grid (each face tile starts / ends on the
+// face); otherwise the closure shells go to the shell kernel.
+inline bool tiles_aligned(const Ctx &C) {
+  const int t[3] = {TX, TY, TZ};
+  for (int a = 0; a < 3; ++a) {
+    const Axis &A = C.ax[a];
+    if (!(A.lbc || A.rbc)) continue;
+    if (A.start % t[a] != 0 || A.end % t[a] != 0) return false;
+  }
+  return true;
+}
+
+template <typename T, bool MAD, bool CL>
+inline int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
+  int rc = BSSN_OK;
+  // lean kernel: every tile, or with in-tile closures only the interior tiles
+  if (parts & (PART_TILES | PART_LEAN)) {
+    const TileSet S = tile_set(C, CL ? 1 : 0, TX, TY, TZ);
+    if (S.count > 0) {
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
+                         O, S);
+      rc = check_hip(hipGetLastError(), "tile kernel launch");
+      if (rc) return rc;
+    }
+  }
+  // closure-capable kernel on the face tiles
+  if (CL && (parts & (PART_TILES | PART_CLOSURE))) {
+    const TileSet S = tile_set(C, 2, TX, TY, TZ);
+    if (S.count > 0) {
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
+                         O, S);
+      rc = check_hip(hipGetLastError(), "tile kernel launch");
+      if (rc) return rc;
+    }
+  }
+  if ((parts & PART_SHELL) && C.any_shell && !CL) {
+    Slabs S = make_slabs(C);
+    const int64_t total = S.first[S.count];
+    if (total > 0) {
+      const int64_t blocks = (total + 255) / 256;
+      hipLaunchKernelGGL((k_stage_shell<T>), dim3((unsigned)blocks), dim3(256), 0, st, C, us, O, S);
+      rc = check_hip(hipGetLastError(), "shell kernel launch");
+    }
+  }
+  return rc;
+}
+
+template <typename T>
+inline int launch_stage_p(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
+  const bool mad = C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0;
+  const bool cl = C.any_shell && tiles_aligned(C);
+  if (mad) return cl ? launch_stage_t<T, true, true>(C, us, O, st, parts) : launch_stage_t<T, true, false>(C, us, O, st, parts);
+  return cl ? launch_stage_t<T, false, true>(C, us, O, st, parts) : launch_stage_t<T, false, false>(C, us, O, st, parts);
+}
+
+// Launchers of one precision (defined in stage_f64.hip / stage_f32.hip).
+int launch_stage_f64(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts);
+int launch_stage_f32(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts);
+// the closure-capable point kernel over a list of slabs
+int launch_slabs_f64(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st);
+int launch_slabs_f32(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st);
+
+template <typename T>
+inline int launch_slabs_t(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st) {
+  const int64_t total = S.first[S.count];
+  if (total <= 0) return BSSN_OK;
+  const int64_t blocks = (total + 255) / 256;
+  hipLaunchKernelGGL((k_stage_shell<T>), dim3((unsigned)blocks), dim3(256), 0, st, C, us, O, S);
+  return check_hip(hipGetLastError(), "point kernel launch");
+}
+
+}  // namespace bssn
