@@ -50,15 +50,20 @@ class BatchEvolver:
         self.plan = _lib.CBatchPlan()
         self.graph = None
         key = (self.shapes, precision)
+        self.tuning = {"source": "given"}
         if tile_variant is None or grid_blocks is None:
             if autotune and key in _TUNED:
                 tile_variant, grid_blocks = _TUNED[key]
+                self.tuning = {"source": "cached"}
             elif autotune:
                 tile_variant, grid_blocks = self.autotune()
                 _TUNED[key] = (tile_variant, grid_blocks)
             else:
                 tile_variant, grid_blocks = 0, 2 * _cu_count()
+                self.tuning = {"source": "default"}
         self.prepare(tile_variant, grid_blocks)
+        self.tuning.update(tile_variant=self.tile_variant, tile_shape=TILE_VARIANTS[self.tile_variant],
+                           grid_blocks=self.grid_blocks)
 
     @property
     def cells(self):
@@ -121,8 +126,7 @@ class BatchEvolver:
                     best, best_t = (v, blocks), t
         for s, v in zip(self.states, saved):
             s.copy_(v)
-        self.tuning = {"tile_variant": best[0], "tile_shape": TILE_VARIANTS[best[0]], "grid_blocks": best[1],
-                       "step_seconds": best_t}
+        self.tuning = {"source": "autotuned", "step_seconds": best_t}
         return best
 
     def variables(self, like=None):
