@@ -179,6 +179,40 @@ def batched_gauge_waves(J, steps, count=64, seed=0, precision="f64"):
             "max_rel_diff_vs_unbatched": worst}
 
 
+def axisymmetric_puncture(J, steps, nr=512, nz=1024, separation=1.5, precision="f64"):
+    """Head-on pair of zero-momentum punctures (m = 0.5 at z = +-s/2) on the
+    compact (rho, z) plane nr x nz (+ 4 parity ghosts), dx = 1/32, moving-
+    puncture gauge (1+log, Gamma driver eta = 2, kappa = 0.002, KO nu = 0.02),
+    Sommerfeld outer faces: cell-updates/s of compact points."""
+    import numpy as np
+    import torch
+
+    from jax_bssn_amd.cartoon import AxisymmetricEvolver
+    from jax_bssn_amd.initial_data import puncture_state
+    dx = 1.0 / 32
+    rho = (np.arange(-4, nr) + 0.5) * dx
+    z = (np.arange(nz) - (nz - 1) / 2.0) * dx
+    R, Z = np.meshgrid(rho, z, indexing="ij")
+    X, Y = R[:, None, :], np.zeros_like(R[:, None, :])
+    v = puncture_state(X, Y, Z[:, None, :], masses=(0.5, 0.5),
+                       positions=((0.0, 0.0, -separation / 2), (0.0, 0.0, separation / 2)))
+    p = J.BSSNParameters(eta=2.0, kappa=0.002, nu=0.02, g=0.75, dx=dx, dt=0.2 * dx, zero_shift=0, gauge=1,
+                         xl_bc=0, xr_bc=1, yl_bc=0, yr_bc=0, zl_bc=1, zr_bc=1, x_min=-3.5 * dx, y_min=-4.0 * dx,
+                         z_min=float(z[0]))
+    ev = AxisymmetricEvolver(v, p, precision=precision)
+    ev.step(1)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ev.step(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3
+    return {"value": nr * nz * steps / t, "unit": "cell-updates/s (compact points)", "compact": [nr, nz],
+            "ms_per_step": 1e3 * t / steps, "precision": precision,
+            "finite": bool(torch.isfinite(ev.u).all())}
+
+
 def copy_calibration(nbytes=1 << 31, reps=3):
     """A plain device copy of known size (read nbytes, write nbytes), for
     scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -224,6 +258,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-cartoon", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -281,6 +316,8 @@ def main():
     }
     if not args.no_batch:
         extra["batched_gauge_waves"] = batched_gauge_waves(J, args.steps)
+    if not args.no_cartoon:
+        extra["axisymmetric_puncture"] = axisymmetric_puncture(J, args.steps)
     line.update(extra)
     if args.copy_calibration:
         copy_calibration()

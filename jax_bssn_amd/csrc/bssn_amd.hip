@@ -230,10 +230,20 @@ __global__ void __launch_bounds__(256) k_derived(int64_t n, const double *__rest
   if (i < n) derived_point(n, i, us, D);
 }
 
+struct Box {
+  int lo[3], hi[3];
+};
+
+// kappa term on the points of the box (the whole grid or a region)
 __global__ void __launch_bounds__(256) k_kappa(Ctx C, const double *__restrict__ us, const double *__restrict__ D,
-                                               const double *__restrict__ matter, double *kap) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < C.ncell) kappa_point(C, us, D, matter, i, kap);
+                                               const double *__restrict__ matter, double *kap, Box B) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ex = B.hi[0] - B.lo[0], ey = B.hi[1] - B.lo[1], ez = B.hi[2] - B.lo[2];
+  if (gid >= (int64_t)ex * ey * ez) return;
+  const int z = B.lo[2] + (int)(gid % ez);
+  const int y = B.lo[1] + (int)((gid / ez) % ey);
+  const int x = B.lo[0] + (int)(gid / ((int64_t)ez * ey));
+  kappa_point(C, us, D, matter, (int64_t)x * C.ax[0].stride + (int64_t)y * C.ax[1].stride + z, kap);
 }
 
 // ------------------------------------------------------------- reductions
@@ -330,13 +340,24 @@ int resolve(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
 // functions and the kappa term when kappa != 0.
 int64_t stage_scratch(const Ctx &C) { return C.kappa != 0.0 ? (int64_t)(NDERIVED + 6) * C.ncell : 0; }
 
-int launch_kappa(const Ctx &C, const double *us, const double *matter, double *work, hipStream_t st) {
+// Derived fields on the whole grid (the kappa stencils reach past the box),
+// then the kappa term on the box [lo, hi) (whole grid when lo is null).
+int launch_kappa(const Ctx &C, const double *us, const double *matter, double *work, hipStream_t st,
+                 const int *lo = nullptr, const int *hi = nullptr) {
   const int64_t blocks = (C.ncell + 255) / 256;
   double *D = work, *kap = work + (int64_t)NDERIVED * C.ncell;
   hipLaunchKernelGGL(k_derived, dim3((unsigned)blocks), dim3(256), 0, st, C.ncell, us, D);
   int rc = check_hip(hipGetLastError(), "derived-field kernel launch");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_kappa, dim3((unsigned)blocks), dim3(256), 0, st, C, us, D, matter, kap);
+  Box B;
+  int64_t pts = 1;
+  for (int a = 0; a < 3; ++a) {
+    B.lo[a] = lo ? lo[a] : 0;
+    B.hi[a] = hi ? hi[a] : C.ax[a].n;
+    pts *= B.hi[a] - B.lo[a];
+  }
+  if (pts <= 0) return BSSN_OK;
+  hipLaunchKernelGGL(k_kappa, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, st, C, us, D, matter, kap, B);
   return check_hip(hipGetLastError(), "kappa kernel launch");
 }
 
@@ -364,7 +385,7 @@ int launch_region(const Ctx &C, int precision, const double *us, double *rhs, co
   stage_update(O, C.dt);
   if (C.kappa != 0.0) {
     if (!work) return fail(BSSN_ERR_INVALID, "kappa != 0 needs a work buffer of bssn_work_size() doubles");
-    const int rc = launch_kappa(C, us, nullptr, work, st);
+    const int rc = launch_kappa(C, us, nullptr, work, st, lo, hi);
     if (rc) return rc;
     O.kap = work + (int64_t)NDERIVED * C.ncell;
   }

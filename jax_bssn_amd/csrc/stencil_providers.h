@@ -407,22 +407,12 @@ struct TileProvider {
     }
   }
 
-  // mixed second derivative: inner centred D1 along a, outer along b
+  // mixed second derivative: inner centred D1 along a, outer along b.  A
+  // closure row is used only along an axis whose zone this lane is in; the
+  // other axis keeps its compact centred (or MAD) formula.
   __device__ __forceinline__ T dmix(int a, int b) const {
     if (red[a] || red[b]) return T(0);
-    if (zone(a, 2) || zone(b, 2)) {
-      const Row8<T> ra = row_d1<T>(ax[a], pos[a], (double)inv_dx, false);
-      const Row8<T> rb = row_d1<T>(ax[b], pos[b], (double)inv_dx, false);
-      T s = T(0);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        T inner = T(0);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) inner += ra.c[i] * at2(a, ra.off(i), b, rb.off(j));
-        s += rb.c[j] * inner;
-      }
-      return s;
-    }
+    // centred inner D1 along a (without 1/dx) at outer offset ob along b
     auto inner = [&](int ob) -> T {
       T f1 = at2(a, 1, b, ob), b1 = at2(a, -1, b, ob);
       T f2 = at2(a, 2, b, ob), b2 = at2(a, -2, b, ob);
@@ -435,15 +425,50 @@ struct TileProvider {
         return q[a] * d4 + (T(1) - q[a]) * d6;
       }
     };
-    T p1 = inner(1), m1 = inner(-1), p2 = inner(2), m2 = inner(-2);
-    T d4 = T(2.0 / 3.0) * (p1 - m1) - T(1.0 / 12.0) * (p2 - m2);
-    if constexpr (!MAD) {
-      return d4 * inv_dx2;
-    } else {
-      T p3 = inner(3), m3 = inner(-3);
-      T d6 = T(3.0 / 4.0) * (p1 - m1) - T(3.0 / 20.0) * (p2 - m2) + T(1.0 / 60.0) * (p3 - m3);
-      return (q[b] * d4 + (T(1) - q[b]) * d6) * inv_dx2;
+    // centred outer D1 along b (without 1/dx) of values g(ob)
+    auto outer = [&](auto &&g) -> T {
+      T p1 = g(1), m1 = g(-1), p2 = g(2), m2 = g(-2);
+      T d4 = T(2.0 / 3.0) * (p1 - m1) - T(1.0 / 12.0) * (p2 - m2);
+      if constexpr (!MAD) {
+        return d4;
+      } else {
+        T p3 = g(3), m3 = g(-3);
+        T d6 = T(3.0 / 4.0) * (p1 - m1) - T(3.0 / 20.0) * (p2 - m2) + T(1.0 / 60.0) * (p3 - m3);
+        return q[b] * d4 + (T(1) - q[b]) * d6;
+      }
+    };
+    const bool za = zone(a, 2), zb = zone(b, 2);
+    if (za && zb) {
+      const Row8<T> ra = row_d1<T>(ax[a], pos[a], (double)inv_dx, false);
+      const Row8<T> rb = row_d1<T>(ax[b], pos[b], (double)inv_dx, false);
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        T in = T(0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in += ra.c[i] * at2(a, ra.off(i), b, rb.off(j));
+        s += rb.c[j] * in;
+      }
+      return s;
     }
+    if (za) {  // closure row along a (1/dx in its weights), compact outer along b
+      const Row8<T> ra = row_d1<T>(ax[a], pos[a], (double)inv_dx, false);
+      auto row_a = [&](int ob) -> T {
+        T in = T(0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in += ra.c[i] * at2(a, ra.off(i), b, ob);
+        return in;
+      };
+      return outer(row_a) * inv_dx;
+    }
+    if (zb) {  // compact inner along a, closure row along b
+      const Row8<T> rb = row_d1<T>(ax[b], pos[b], (double)inv_dx, false);
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += rb.c[j] * inner(rb.off(j));
+      return s * inv_dx;
+    }
+    return outer(inner) * inv_dx2;
   }
 
   // upwind derivative selected by the sign of the RHS coefficient c
