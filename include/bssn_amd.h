@@ -261,6 +261,27 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
                        bssn_batch_plan *plan, void *stream);
 int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream);
 
+/* Forward-mode differentiation (no reference counterpart: the reference
+ * differentiates with jax autodiff).  Exact tangent-linear maps of the
+ * operators above along a direction dstate (same layout as state), kappa = 0
+ * and vacuum only (BSSN_ERR_UNSUPPORTED otherwise):
+ *
+ * bssn_rhs_tangent: rhs <- RHS(state), drhs <- dRHS(state)[dstate]
+ * (compute_bssn_rhs, time_evolve.py:41-57). */
+int bssn_rhs_tangent(const bssn_params *p, const bssn_grid *grid, const double *state,
+                     const double *dstate, double *rhs, double *drhs, void *stream);
+
+/* bssn_evolve with the tangent carried along: state <- rk4_step^nsteps
+ * (state), dstate <- its derivative along dstate, both in place.  work holds
+ * 6 state-sized arrays (6 * 21 * n doubles). */
+int bssn_evolve_tangent(const bssn_params *p, const bssn_grid *grid, int nsteps, double *state,
+                        double *dstate, double *work, void *stream);
+
+/* The Hamiltonian constraint (bssn/constraints.py:271-303) and its tangent:
+ * H[n], dH[n]. */
+int bssn_hamiltonian_tangent(const bssn_params *p, const bssn_grid *grid, const double *state,
+                             const double *dstate, double *H, double *dH, void *stream);
+
 /* Launch parts of one RK4 stage, for the measurement helpers below:
  * TILES    both tile kernels (every point the tile grid owns);
  * SHELL    the Sommerfeld closure-shell kernel (tiles not aligned to faces);

@@ -84,8 +84,8 @@ BSSN_HD __forceinline__ T advection(const P &pv, const T (&b)[3]) {
 // and every component's RHS is replaced by the radial Sommerfeld RHS
 // (reference evolution/boundaries.py:75-172) via out.put_face().
 // u[] are the point values.
-template <typename T, class P, class E>
-BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&u)[NCOMP], E &out) {
+template <typename T, class P, class E, typename V>
+BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NCOMP], E &out) {
   T g[6], A[6], G[3], b[3];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -115,7 +115,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     pv.stage(i);
-    if (face) out.put_face(BETA + i, (double)pv.somm(BETA + i, u[BETA + i]));
+    if (face) out.put_face(BETA + i, V(pv.somm(BETA + i, u[BETA + i])));
 #pragma unroll
     for (int j = 0; j < 3; ++j) db[i][j] = pv.d1(j);
     T s[6];
@@ -128,7 +128,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     }
     const T tr = advection(pv, b) + pv.ko();
     const T rb = C.zero_shift ? T(0) * b[i] : (T)C.g * G[i] + (tr - (T)C.eta * b[i]);
-    out.put(BETA + i, (double)rb);
+    out.put(BETA + i, V(rb));
   }
   const T divb = db[0][0] + db[1][1] + db[2][2];
   // Lie terms of A and the shift terms of the connection, while d beta is live
@@ -138,7 +138,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     T lie = T(0);
 #pragma unroll
     for (int m = 0; m < 3; ++m) lie += A[sym(m, i)] * db[m][j] + A[sym(m, j)] * db[m][i];
-    pv.st.put(ST_ALIE + c, (double)(lie - T(2.0 / 3.0) * A[c] * divb));
+    pv.st.put(ST_ALIE + c, V(lie - T(2.0 / 3.0) * A[c] * divb));
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -149,7 +149,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     pv.stage(3 + c);
-    if (face) out.put_face(GXX + c, (double)pv.somm(GXX + c, u[GXX + c]));
+    if (face) out.put_face(GXX + c, V(pv.somm(GXX + c, u[GXX + c])));
 #pragma unroll
     for (int d = 0; d < 3; ++d) dg[c][d] = pv.d1(d);
     T s[6];
@@ -159,18 +159,18 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     T lie = T(0);
 #pragma unroll
     for (int m = 0; m < 3; ++m) lie += g[sym(m, i)] * db[m][j] + g[sym(m, j)] * db[m][i];
-    out.put(GXX + c, (double)(advection(pv, b) + lie - T(2.0 / 3.0) * g[c] * divb - T(2) * al * A[c] + pv.ko()));
+    out.put(GXX + c, V(advection(pv, b) + lie - T(2.0 / 3.0) * g[c] * divb - T(2) * al * A[c] + pv.ko()));
   }
 
   // A and the shift are idle until after the Christoffel algebra
 #pragma unroll
-  for (int c = 0; c < 6; ++c) pv.st.put(ST_A + c, (double)A[c]);
+  for (int c = 0; c < 6; ++c) pv.st.put(ST_A + c, V(A[c]));
 
   // ---- lapse (visit 9): d_i alpha and W^2 d_i d_j alpha
   T da[3], Hs[6];
   {
     pv.stage(9);
-    if (face) out.put_face(ALPHA, (double)pv.somm(ALPHA, u[ALPHA]));
+    if (face) out.put_face(ALPHA, V(pv.somm(ALPHA, u[ALPHA])));
 #pragma unroll
     for (int j = 0; j < 3; ++j) da[j] = pv.d1(j);
     T s[6];
@@ -178,14 +178,14 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
 #pragma unroll
     for (int c = 0; c < 6; ++c) Hs[c] = W2 * s[c];
     const T slicing = (C.gauge == 0) ? -(al * al) * K : T(-2) * al * K;
-    out.put(ALPHA, (double)(slicing + advection(pv, b) + pv.ko()));
+    out.put(ALPHA, V(slicing + advection(pv, b) + pv.ko()));
   }
 
   // ---- conformal factor (visit 10): W (d_i d_j W + gamma_ij gamma^mn d_m d_n W)
   T dW[3];
   {
     pv.stage(10);
-    if (face) out.put_face(CW, (double)pv.somm(CW, u[CW]));
+    if (face) out.put_face(CW, V(pv.somm(CW, u[CW])));
 #pragma unroll
     for (int j = 0; j < 3; ++j) dW[j] = pv.d1(j);
     T s[6];
@@ -193,27 +193,27 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     const T lap = contract(ig, s);
 #pragma unroll
     for (int c = 0; c < 6; ++c) R[c] += W * (s[c] + g[c] * lap);
-    out.put(CW, (double)(advection(pv, b) + T(1.0 / 3.0) * al * W * K - T(1.0 / 3.0) * W * divb + pv.ko()));
+    out.put(CW, V(advection(pv, b) + T(1.0 / 3.0) * al * W * K - T(1.0 / 3.0) * W * divb + pv.ko()));
   }
 
   // ---- trace K (visit 11): -4/3 alpha gamma^ij d_j K goes straight to the connection
   {
     pv.stage(11);
-    if (face) out.put_face(CK, (double)pv.somm(CK, u[CK]));
+    if (face) out.put_face(CK, V(pv.somm(CK, u[CK])));
     T dK[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) dK[j] = pv.d1(j);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       rG[i] += T(-4.0 / 3.0) * al * (ig[sym(i, 0)] * dK[0] + ig[sym(i, 1)] * dK[1] + ig[sym(i, 2)] * dK[2]);
-    pv.st.put(ST_RK, (double)(advection(pv, b) + pv.ko()));
+    pv.st.put(ST_RK, V(advection(pv, b) + pv.ko()));
   }
 
   // ---- connection (visits 12..14): W^2 gamma_m(i d_j) Gamma^m
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     pv.stage(12 + m);
-    if (face) out.put_face(GAM + m, (double)pv.somm(GAM + m, u[GAM + m]));
+    if (face) out.put_face(GAM + m, V(pv.somm(GAM + m, u[GAM + m])));
     T dG[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) dG[j] = pv.d1(j);
@@ -225,7 +225,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
     rG[m] += advection(pv, b) + pv.ko();
   }
 #pragma unroll
-  for (int i = 0; i < 3; ++i) pv.st.put(ST_B + i, (double)b[i]);
+  for (int i = 0; i < 3; ++i) pv.st.put(ST_B + i, V(b[i]));
 
   // ---- Christoffel algebra.  C1[i][jk] = Gamma_ijk.  For each upper index k
   // the row Gamma^k_ij = gamma^km Gamma_mij is formed, used and dropped.
@@ -319,7 +319,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
       if (out.has_matter())
         src = T(-16.0 * M_PI) * al *
               (ig[sym(i, 0)] * (T)out.matter(1) + ig[sym(i, 1)] * (T)out.matter(2) + ig[sym(i, 2)] * (T)out.matter(3));
-      out.put(GAM + i, (double)(rG[i] + T(2) * al * t2 - six_al_W * t3 - T(2) * t4 + src));
+      out.put(GAM + i, V(rG[i] + T(2) * al * t2 - six_al_W * t3 - T(2) * t4 + src));
     }
   }
 
@@ -333,7 +333,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
       for (int c = 0; c < 6; ++c) S[c] = (T)out.matter(4 + c);
       rKv += T(4.0 * M_PI) * al * ((T)out.matter(0) + Wf * Wf * contract(ig, S));
     }
-    out.put(CK, (double)rKv);
+    out.put(CK, V(rKv));
   }
 
   // ---- everything of the A RHS except transport and dissipation
@@ -367,8 +367,8 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const double (&
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     pv.stage(15 + c);
-    if (face) out.put_face(AXX + c, (double)pv.somm(AXX + c, u[AXX + c]));
-    out.put(AXX + c, (double)(Arest[c] + advection(pv, b) + pv.ko()) + out.extra_A(c));
+    if (face) out.put_face(AXX + c, V(pv.somm(AXX + c, u[AXX + c])));
+    out.put(AXX + c, V(Arest[c] + advection(pv, b) + pv.ko()) + out.extra_A(c));
   }
   out.done();
 }

@@ -19,17 +19,18 @@ namespace bssn {
 
 // Reference time_evolve.py:94-131: gamma <- gamma det^(-1/3); then
 // A <- A - 1/3 gamma (gamma^ij A_ij) with the inverse of the rescaled metric.
-BSSN_HD __forceinline__ void project(double (&g)[6], double (&A)[6]) {
-  const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
-                     g[2] * (g[1] * g[4] - g[3] * g[2]);
-  const double s = 1.0 / cbrt(det);
+template <typename V>
+BSSN_HD __forceinline__ void project(V (&g)[6], V (&A)[6]) {
+  const V det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
+                g[2] * (g[1] * g[4] - g[3] * g[2]);
+  const V s = V(1.0) / cbrt(det);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) g[c] *= s;
-  double ig[6];
+  for (int c = 0; c < 6; ++c) g[c] = g[c] * s;
+  V ig[6];
   inverse_metric(g, ig);
-  const double tr = contract(ig, A);
+  const V tr = contract(ig, A);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) A[c] -= (1.0 / 3.0) * g[c] * tr;
+  for (int c = 0; c < 6; ++c) A[c] = A[c] - V(1.0 / 3.0) * g[c] * tr;
 }
 
 // ------------------------------------------------------------ stage update
