@@ -47,10 +47,20 @@ __global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
   using P = TileProvider<T, TX, TY, TZ, MAD, CL>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
-  int bx, by, bz;
-  tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz);
+  int bx, by, bz, slab;
+  tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz, &slab);
   const int tid = threadIdx.x;
-  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  // Waves are planes across the slowest of the three thread coordinates.
+  // A face tile puts that coordinate along its closure axis, so a wave's
+  // lanes share (or nearly share) one closure row instead of diverging.
+  int tx, ty, tz;
+  if (!CL || slab == 0) {
+    tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  } else if (slab == 1) {
+    tz = tid % TZ, tx = (tid / TZ) % TX, ty = tid / (TX * TZ);
+  } else {
+    ty = tid % TY, tx = (tid / TY) % TX, tz = tid / (TX * TY);
+  }
   const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
   const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
   const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;

@@ -291,7 +291,7 @@ struct TileProvider {
     cur = lds;
     src = (const BSSN_GLOBAL double *)u;
     ncell = C.ncell;
-    tid = (tx * TY + ty) * TZ + tz;
+    tid = threadIdx.x;  // the cooperative loads and the stash follow the lanes, not the point
     st.base = (BSSN_LDS double *)stash_mem + tid;
     c = ((tx + HX) * LY + (ty + HY)) * LZ + (tz + HZ);
     inv_dx = (T)C.inv_dx;

@@ -36,7 +36,10 @@ BSSN_HD __forceinline__ int compl_pick(const TileSet &S, int a, int j) {
 }
 
 // Tile coordinates of the k-th tile of the set, z fastest.
-BSSN_HD __forceinline__ void tile_of(const TileSet &S, int k, int &bx, int &by, int &bz) {
+// ``slab`` (optional) receives the closure slab of a face tile: 0 x, 1 y,
+// 2 z (0 for the other kinds).
+BSSN_HD __forceinline__ void tile_of(const TileSet &S, int k, int &bx, int &by, int &bz, int *slab = nullptr) {
+  if (slab) *slab = 0;
   if (S.kind == 0) {
     bz = k % S.n[2];
     by = (k / S.n[2]) % S.n[1];
@@ -57,10 +60,12 @@ BSSN_HD __forceinline__ void tile_of(const TileSet &S, int k, int &bx, int &by, 
     by = (k / S.n[2]) % S.n[1];
     bx = range_pick(S, 0, k / (S.n[2] * S.n[1]));
   } else if ((k -= nx_slab) < ny_slab) {
+    if (slab) *slab = 1;
     bz = k % S.n[2];
     by = range_pick(S, 1, (k / S.n[2]) % sy);
     bx = compl_pick(S, 0, k / (S.n[2] * sy));
   } else {
+    if (slab) *slab = 2;
     k -= ny_slab;
     const int sz = range_size(S, 2);
     bz = range_pick(S, 2, k % sz);
