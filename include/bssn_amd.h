@@ -263,23 +263,27 @@ int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream);
 
 /* Forward-mode differentiation (no reference counterpart: the reference
  * differentiates with jax autodiff).  Exact tangent-linear maps of the
- * operators above along a direction dstate (same layout as state), kappa = 0
- * and vacuum only (BSSN_ERR_UNSUPPORTED otherwise):
+ * operators above along ndir = 1 or 2 directions at once, kappa = 0 and
+ * vacuum only (BSSN_ERR_UNSUPPORTED otherwise).  A tangent buffer holds
+ * ndir state-layout arrays back to back (ndir * 21 * n doubles; direction l
+ * at offset l * 21 * n):
  *
  * bssn_rhs_tangent: rhs <- RHS(state), drhs <- dRHS(state)[dstate]
  * (compute_bssn_rhs, time_evolve.py:41-57). */
-int bssn_rhs_tangent(const bssn_params *p, const bssn_grid *grid, const double *state,
+int bssn_rhs_tangent(const bssn_params *p, const bssn_grid *grid, int ndir, const double *state,
                      const double *dstate, double *rhs, double *drhs, void *stream);
 
-/* bssn_evolve with the tangent carried along: state <- rk4_step^nsteps
- * (state), dstate <- its derivative along dstate, both in place.  work holds
- * 6 state-sized arrays (6 * 21 * n doubles). */
-int bssn_evolve_tangent(const bssn_params *p, const bssn_grid *grid, int nsteps, double *state,
+/* Doubles of scratch bssn_evolve_tangent needs: 3 (1 + ndir) * 21 * n. */
+int64_t bssn_tangent_work_size(const bssn_grid *grid, int ndir);
+
+/* bssn_evolve with the tangents carried along: state <- rk4_step^nsteps
+ * (state), dstate <- its derivatives along dstate, both in place. */
+int bssn_evolve_tangent(const bssn_params *p, const bssn_grid *grid, int nsteps, int ndir, double *state,
                         double *dstate, double *work, void *stream);
 
-/* The Hamiltonian constraint (bssn/constraints.py:271-303) and its tangent:
- * H[n], dH[n]. */
-int bssn_hamiltonian_tangent(const bssn_params *p, const bssn_grid *grid, const double *state,
+/* The Hamiltonian constraint (bssn/constraints.py:271-303) and its
+ * tangents: H[n], dH[ndir * n]. */
+int bssn_hamiltonian_tangent(const bssn_params *p, const bssn_grid *grid, int ndir, const double *state,
                              const double *dstate, double *H, double *dH, void *stream);
 
 /* Launch parts of one RK4 stage, for the measurement helpers below:

@@ -1,8 +1,9 @@
-// Forward-mode (tangent-linear) arithmetic: a value and its derivative along
-// one direction.  Instantiating the per-point RHS, the stage update and the
-// projection with Dual yields the exact Jacobian-vector product of the RK4
-// step (no finite differences).  Comparisons and branches (upwind sign,
-// W floor) follow the value, as the derivative of the branch taken.
+// Forward-mode (tangent-linear) arithmetic: a value and its derivatives
+// along ND directions.  Instantiating the per-point RHS, the stage update
+// and the projection with DualN yields the exact Jacobian-vector products
+// of the RK4 step along ND directions at once (no finite differences).
+// Comparisons and branches (upwind sign, W floor) follow the value, as the
+// derivative of the branch taken.
 #pragma once
 
 #include <cmath>
@@ -11,42 +12,139 @@
 
 namespace bssn {
 
-struct Dual {
-  double v, d;
-  Dual() = default;
-  BSSN_HD constexpr Dual(double x) : v(x), d(0.0) {}
-  BSSN_HD constexpr Dual(double x, double dx) : v(x), d(dx) {}
-  BSSN_HD Dual &operator+=(const Dual &o) { v += o.v; d += o.d; return *this; }
-  BSSN_HD Dual &operator-=(const Dual &o) { v -= o.v; d -= o.d; return *this; }
-  BSSN_HD Dual &operator*=(const Dual &o) { d = d * o.v + v * o.d; v *= o.v; return *this; }
+template <int ND>
+struct DualN {
+  double v, d[ND];
+  DualN() = default;
+  BSSN_HD constexpr DualN(double x) : v(x), d{} {}
+  BSSN_HD DualN &operator+=(const DualN &o) {
+    v += o.v;
+#pragma unroll
+    for (int l = 0; l < ND; ++l) d[l] += o.d[l];
+    return *this;
+  }
+  BSSN_HD DualN &operator-=(const DualN &o) {
+    v -= o.v;
+#pragma unroll
+    for (int l = 0; l < ND; ++l) d[l] -= o.d[l];
+    return *this;
+  }
+  BSSN_HD DualN &operator*=(const DualN &o) {
+#pragma unroll
+    for (int l = 0; l < ND; ++l) d[l] = d[l] * o.v + v * o.d[l];
+    v *= o.v;
+    return *this;
+  }
 };
 
-BSSN_HD __forceinline__ Dual operator+(const Dual &a, const Dual &b) { return Dual(a.v + b.v, a.d + b.d); }
-BSSN_HD __forceinline__ Dual operator-(const Dual &a, const Dual &b) { return Dual(a.v - b.v, a.d - b.d); }
-BSSN_HD __forceinline__ Dual operator-(const Dual &a) { return Dual(-a.v, -a.d); }
-BSSN_HD __forceinline__ Dual operator*(const Dual &a, const Dual &b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
-BSSN_HD __forceinline__ Dual operator/(const Dual &a, const Dual &b) {
-  const double r = 1.0 / b.v, q = a.v * r;
-  return Dual(q, (a.d - q * b.d) * r);
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator+(const DualN<ND> &a, const DualN<ND> &b) {
+  DualN<ND> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = a.d[l] + b.d[l];
+  return r;
 }
-BSSN_HD __forceinline__ Dual operator+(const Dual &a, double b) { return Dual(a.v + b, a.d); }
-BSSN_HD __forceinline__ Dual operator+(double a, const Dual &b) { return Dual(a + b.v, b.d); }
-BSSN_HD __forceinline__ Dual operator-(const Dual &a, double b) { return Dual(a.v - b, a.d); }
-BSSN_HD __forceinline__ Dual operator-(double a, const Dual &b) { return Dual(a - b.v, -b.d); }
-BSSN_HD __forceinline__ Dual operator*(const Dual &a, double b) { return Dual(a.v * b, a.d * b); }
-BSSN_HD __forceinline__ Dual operator*(double a, const Dual &b) { return Dual(a * b.v, a * b.d); }
-BSSN_HD __forceinline__ Dual operator/(const Dual &a, double b) { return Dual(a.v / b, a.d / b); }
-BSSN_HD __forceinline__ bool operator<(const Dual &a, const Dual &b) { return a.v < b.v; }
-BSSN_HD __forceinline__ bool operator>(const Dual &a, const Dual &b) { return a.v > b.v; }
-BSSN_HD __forceinline__ bool operator<=(const Dual &a, const Dual &b) { return a.v <= b.v; }
-BSSN_HD __forceinline__ bool operator>=(const Dual &a, const Dual &b) { return a.v >= b.v; }
-BSSN_HD __forceinline__ Dual sqrt(const Dual &a) {
-  const double s = ::sqrt(a.v);
-  return Dual(s, a.d * 0.5 / s);
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator-(const DualN<ND> &a, const DualN<ND> &b) {
+  DualN<ND> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = a.d[l] - b.d[l];
+  return r;
 }
-BSSN_HD __forceinline__ Dual cbrt(const Dual &a) {
-  const double c = ::cbrt(a.v);
-  return Dual(c, a.d * c / (3.0 * a.v));
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator-(const DualN<ND> &a) {
+  DualN<ND> r;
+  r.v = -a.v;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = -a.d[l];
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator*(const DualN<ND> &a, const DualN<ND> &b) {
+  DualN<ND> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = a.d[l] * b.v + a.v * b.d[l];
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator/(const DualN<ND> &a, const DualN<ND> &b) {
+  const double s = 1.0 / b.v;
+  DualN<ND> r;
+  r.v = a.v * s;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = (a.d[l] - r.v * b.d[l]) * s;
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator+(const DualN<ND> &a, double b) {
+  DualN<ND> r = a;
+  r.v += b;
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator+(double a, const DualN<ND> &b) {
+  return b + a;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator-(const DualN<ND> &a, double b) {
+  DualN<ND> r = a;
+  r.v -= b;
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator-(double a, const DualN<ND> &b) {
+  DualN<ND> r = -b;
+  r.v += a;
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator*(const DualN<ND> &a, double b) {
+  DualN<ND> r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = a.d[l] * b;
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator*(double a, const DualN<ND> &b) {
+  return b * a;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> operator/(const DualN<ND> &a, double b) {
+  DualN<ND> r;
+  r.v = a.v / b;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = a.d[l] / b;
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ bool operator<(const DualN<ND> &a, const DualN<ND> &b) { return a.v < b.v; }
+template <int ND>
+BSSN_HD __forceinline__ bool operator>(const DualN<ND> &a, const DualN<ND> &b) { return a.v > b.v; }
+template <int ND>
+BSSN_HD __forceinline__ bool operator<=(const DualN<ND> &a, const DualN<ND> &b) { return a.v <= b.v; }
+template <int ND>
+BSSN_HD __forceinline__ bool operator>=(const DualN<ND> &a, const DualN<ND> &b) { return a.v >= b.v; }
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> sqrt(const DualN<ND> &a) {
+  DualN<ND> r;
+  r.v = ::sqrt(a.v);
+  const double h = 0.5 / r.v;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = a.d[l] * h;
+  return r;
+}
+template <int ND>
+BSSN_HD __forceinline__ DualN<ND> cbrt(const DualN<ND> &a) {
+  DualN<ND> r;
+  r.v = ::cbrt(a.v);
+  const double h = r.v / (3.0 * a.v);
+#pragma unroll
+  for (int l = 0; l < ND; ++l) r.d[l] = a.d[l] * h;
+  return r;
 }
 
 }  // namespace bssn

@@ -286,7 +286,7 @@ struct TileProvider {
   }
 
   __device__ __forceinline__ void setup(const Ctx &C, const double *u, T *smem, double *stash_mem, int x0, int y0, int z0,
-                        int tx, int ty, int tz) {
+                        int tx, int ty, int tz, bool prefetch = true) {
     lds = (BSSN_LDS T *)smem;
     cur = lds;
     src = (const BSSN_GLOBAL double *)u;
@@ -332,7 +332,7 @@ struct TileProvider {
       goff[k] = gx * C.ax[0].stride + gy * C.ax[1].stride + gz;
     }
     boff = x0 * C.ax[0].stride + y0 * C.ax[1].stride + z0;
-    fetch(visit_comp(0));
+    if (prefetch) fetch(visit_comp(0));
   }
 
   // Stage visit v: store the prefetched tile, one barrier, prefetch v + 1.

@@ -1,44 +1,37 @@
 // Forward-mode differentiation of the evolution: the exact tangent-linear
-// RK4 step and the tangent of the Hamiltonian constraint.
+// RK4 step along ND = 1 or 2 directions at once, and the tangent of the
+// Hamiltonian constraint.
 //
 // The per-point RHS (rhs_point.h), the stage update and the projection are
-// the same templates the evolution kernels use, instantiated with Dual
-// (value, derivative along one direction).  Stencil taps come straight
-// from global memory (the value state and its tangent), so every point,
-// closure and Sommerfeld face included, runs one kernel.  With p initial-
-// data parameters, p tangent runs give the exact gradient of any output of
-// the evolution (the torch.autograd.Function in jax_bssn_amd/tangent.py).
-// Restricted to kappa = 0 and vacuum.
-
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdint>
-
-#include "bssn_amd.h"
-#include "bssn_common.h"
-#include "dual.h"
-#include "host_errors.h"
-#include "rhs_point.h"
-#include "stage_common.h"
-#include "stencil_providers.h"
+// the same templates the evolution kernels use, instantiated with DualN
+// (value and ND directional derivatives).  Stencils are linear, so a
+// derivative of a dual field is the dual of the value's and the tangents'
+// derivatives: the tile kernel stages the value and the ND tangents of a
+// component in LDS side by side and runs the ordinary tile stencils on each
+// (the upwind side follows the value's shift).  Configurations the tile
+// kernels do not cover (MAD blend, closure faces off the tile grid) take the
+// point kernel, whose taps come straight from global memory.  With p
+// initial-data parameters, ceil(p / 2) tangent runs give the exact gradient
+// of any output of the evolution (the torch.autograd.Function in
+// jax_bssn_amd/tangent.py).  Restricted to kappa = 0 and vacuum.
+//
+// This unit holds the point kernels, the projection, the Hamiltonian
+// tangent and the entry points; the tile kernels are in tangent_common.h.
+#include "tangent_common.h"
 
 namespace bssn {
 namespace {
 
-template <typename V>
-struct StashT {
-  V v[STASH_SLOTS];
-  BSSN_HD __forceinline__ void put(int slot, V x) { v[slot] = x; }
-  BSSN_HD __forceinline__ V get(int slot) const { return v[slot]; }
-};
-
-// Stencil taps of a value state and its tangent, through the same rows
-// (closures, MAD blend, upwind replacement) as PointProvider.
+// Stencil taps of a value state and its ND tangents (direction l at
+// dsrc + l * dstride), through the same rows (closures, MAD blend, upwind
+// replacement) as PointProvider.
+template <int ND>
 struct DualPointProvider {
-  StashT<Dual> st;
+  using V = DualN<ND>;
+  StashT<V> st;
   const Ctx *C;
   const double *src, *dsrc, *f, *df;
+  int64_t dstride;
   int p[3];
   int64_t idx;
   bool isface;
@@ -47,6 +40,7 @@ struct DualPointProvider {
     C = &ctx;
     src = f = u;
     dsrc = df = du;
+    dstride = (int64_t)NCOMP * ctx.ncell;
     p[0] = x, p[1] = y, p[2] = z;
     idx = (int64_t)x * ctx.ax[0].stride + (int64_t)y * ctx.ax[1].stride + z;
     bool fc = false, owned = true;
@@ -63,61 +57,67 @@ struct DualPointProvider {
     df = dsrc + comp * C->ncell;
   }
   __device__ __forceinline__ void stage(int v) { stage_comp(visit_comp(v)); }
-  __device__ __forceinline__ Dual at_index(int64_t j) const { return Dual(f[j], df[j]); }
-  __device__ __forceinline__ Dual ld(int a, int oa) const {
+  __device__ __forceinline__ V at_index(int64_t j) const {
+    V r;
+    r.v = f[j];
+#pragma unroll
+    for (int l = 0; l < ND; ++l) r.d[l] = df[l * dstride + j];
+    return r;
+  }
+  __device__ __forceinline__ V ld(int a, int oa) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
     return at_index((int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]);
   }
-  __device__ __forceinline__ Dual ld2(int a, int oa, int b, int ob) const {
+  __device__ __forceinline__ V ld2(int a, int oa, int b, int ob) const {
     int q[3] = {p[0], p[1], p[2]};
     q[a] = wrap_index(q[a] + oa, C->ax[a].n);
     q[b] = wrap_index(q[b] + ob, C->ax[b].n);
     return at_index((int64_t)q[0] * C->ax[0].stride + (int64_t)q[1] * C->ax[1].stride + q[2]);
   }
-  __device__ __forceinline__ Dual apply(int a, const Row8<double> &r) const {
-    Dual s(0.0);
+  __device__ __forceinline__ V apply(int a, const Row8<double> &r) const {
+    V s(0.0);
     for (int k = 0; k < r.n; ++k)
       if (r.c[k] != 0.0) s += r.c[k] * ld(a, r.o0 + k);
     return s;
   }
-  __device__ __forceinline__ Dual val() const { return at_index(idx); }
-  __device__ __forceinline__ Dual d1(int a) const {
-    if (C->ax[a].reduced) return Dual(0.0);
+  __device__ __forceinline__ V val() const { return at_index(idx); }
+  __device__ __forceinline__ V d1(int a) const {
+    if (C->ax[a].reduced) return V(0.0);
     return apply(a, row_d1<double>(C->ax[a], p[a], C->inv_dx, false));
   }
-  __device__ __forceinline__ Dual d2(int a) const {
-    if (C->ax[a].reduced) return Dual(0.0);
+  __device__ __forceinline__ V d2(int a) const {
+    if (C->ax[a].reduced) return V(0.0);
     return apply(a, row_d2<double>(C->ax[a], p[a], C->inv_dx2));
   }
-  __device__ __forceinline__ Dual dmix(int a, int b) const {
-    if (C->ax[a].reduced || C->ax[b].reduced) return Dual(0.0);
+  __device__ __forceinline__ V dmix(int a, int b) const {
+    if (C->ax[a].reduced || C->ax[b].reduced) return V(0.0);
     const Row8<double> ra = row_d1<double>(C->ax[a], p[a], C->inv_dx, false);
     const Row8<double> rb = row_d1<double>(C->ax[b], p[b], C->inv_dx, false);
-    Dual s(0.0);
+    V s(0.0);
     for (int j = 0; j < rb.n; ++j) {
       if (rb.c[j] == 0.0) continue;
-      Dual in(0.0);
+      V in(0.0);
       for (int i = 0; i < ra.n; ++i)
         if (ra.c[i] != 0.0) in += ra.c[i] * ld2(a, ra.o0 + i, b, rb.o0 + j);
       s += rb.c[j] * in;
     }
     return s;
   }
-  __device__ __forceinline__ Dual upw(int a, Dual coeff) const {
-    if (C->ax[a].reduced) return Dual(0.0);
+  __device__ __forceinline__ V upw(int a, V coeff) const {
+    if (C->ax[a].reduced) return V(0.0);
     return apply(a, row_upwind<double>(C->ax[a], p[a], C->inv_dx, coeff.v));
   }
-  __device__ __forceinline__ Dual ko() const {
-    Dual s(0.0);
+  __device__ __forceinline__ V ko() const {
+    V s(0.0);
     for (int a = 0; a < 3; ++a) {
       if (C->ax[a].reduced) continue;
       s += apply(a, row_d6<double>(C->ax[a], p[a]));
     }
     return C->ko * s;
   }
-  __device__ __forceinline__ Dual somm(int comp, Dual uc) const {
-    const Dual d0 = d1(0), d1v = d1(1), d2v = d1(2);
+  __device__ __forceinline__ V somm(int comp, V uc) const {
+    const V d0 = d1(0), d1v = d1(1), d2v = d1(2);
     const double X = C->xmin[0] + C->dx * (double)p[0];
     const double Y = C->xmin[1] + C->dx * (double)p[1];
     const double Z = C->xmin[2] + C->dx * (double)p[2];
@@ -129,71 +129,7 @@ struct DualPointProvider {
   }
 };
 
-// Tangent buffers of one stage (the value buffers and weights are a StageOut).
-struct TangentOut {
-  const double *du0;
-  double *dustar, *dunext, *drhs;
-};
-
-struct DualEmitter {
-  int64_t n, idx;
-  double wn, ws;
-  const double *nbase, *sbase, *dnbase, *dsbase;
-  double *ustar, *unext, *rhs, *dustar, *dunext, *drhs;
-  int mode, reproject;
-  bool active, face;
-  Dual gn[6], An[6];
-
-  __device__ __forceinline__ DualEmitter(const Ctx &C, const StageOut &O, const TangentOut &T, int64_t i,
-                                         bool on_face)
-      : n(C.ncell), idx(i), wn(O.wn), ws(O.ws), nbase(O.nbase), sbase(O.sbase), ustar(O.ustar), unext(O.unext),
-        rhs(O.rhs), dustar(T.dustar), dunext(T.dunext), drhs(T.drhs), mode(O.mode), reproject(O.reproject),
-        active(true), face(on_face) {
-    dnbase = O.mode == 4 ? T.dustar : T.du0;
-    dsbase = O.mode == 1 ? T.du0 : ((O.mode == 2 || O.mode == 3) ? T.dustar : nullptr);
-  }
-  __device__ __forceinline__ void put(int c, Dual k) {
-    if (!face) write(c, k);
-  }
-  __device__ __forceinline__ void put_face(int c, Dual k) { write(c, k); }
-  __device__ __forceinline__ void write(int c, Dual k) {
-    const int64_t at = c * n + idx;
-    if (mode == 0) {
-      rhs[at] = k.v;
-      drhs[at] = k.d;
-      return;
-    }
-    const Dual un = Dual(nbase[at], dnbase[at]) + wn * k;
-    if (sbase) {
-      const Dual us = Dual(sbase[at], dsbase[at]) + ws * k;
-      ustar[at] = us.v;
-      dustar[at] = us.d;
-    }
-    if (c <= GZZ) {
-      gn[c] = un;
-    } else if (c >= AXX && c <= AZZ) {
-      An[c - AXX] = un;
-    } else {
-      unext[at] = un.v;
-      dunext[at] = un.d;
-    }
-  }
-  __device__ __forceinline__ void done() {
-    if (mode == 0) return;
-    project(gn, An);
-    if (mode == 4 && reproject) project(gn, An);
-    for (int c = 0; c < 6; ++c) {
-      unext[(GXX + c) * n + idx] = gn[c].v;
-      dunext[(GXX + c) * n + idx] = gn[c].d;
-      unext[(AXX + c) * n + idx] = An[c].v;
-      dunext[(AXX + c) * n + idx] = An[c].d;
-    }
-  }
-  __device__ __forceinline__ bool has_matter() const { return false; }
-  __device__ __forceinline__ double matter(int) const { return 0.0; }
-  __device__ __forceinline__ double extra_A(int) const { return 0.0; }
-};
-
+template <int ND>
 __global__ void __launch_bounds__(128) k_stage_tangent(Ctx C, const double *__restrict__ us,
                                                        const double *__restrict__ dus, StageOut O, TangentOut T) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -201,46 +137,65 @@ __global__ void __launch_bounds__(128) k_stage_tangent(Ctx C, const double *__re
   const int z = (int)(i % C.ax[2].n);
   const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
   const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
-  DualPointProvider pv;
+  DualPointProvider<ND> pv;
   pv.setup(C, us, dus, x, y, z);
-  Dual u[NCOMP];
-  for (int c = 0; c < NCOMP; ++c) u[c] = Dual(us[c * C.ncell + i], dus[c * C.ncell + i]);
-  DualEmitter out(C, O, T, i, pv.face());
-  bssn_rhs_point<Dual>(pv, C, u, out);
+  DualN<ND> u[NCOMP];
+  load_dual(C, us, dus, i, u);
+  DualEmitter<ND> out(C, O, T, i, true, pv.face());
+  bssn_rhs_point<DualN<ND>>(pv, C, u, out);
 }
 
+template <int ND>
 __global__ void __launch_bounds__(256) k_project_tangent(int64_t n, double *u, double *du) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Dual g[6], A[6];
+  const int64_t ds = (int64_t)NCOMP * n;
+  DualN<ND> g[6], A[6];
   for (int c = 0; c < 6; ++c) {
-    g[c] = Dual(u[(GXX + c) * n + i], du[(GXX + c) * n + i]);
-    A[c] = Dual(u[(AXX + c) * n + i], du[(AXX + c) * n + i]);
+    g[c].v = u[(GXX + c) * n + i];
+    A[c].v = u[(AXX + c) * n + i];
+#pragma unroll
+    for (int l = 0; l < ND; ++l) {
+      g[c].d[l] = du[l * ds + (GXX + c) * n + i];
+      A[c].d[l] = du[l * ds + (AXX + c) * n + i];
+    }
   }
   project(g, A);
   for (int c = 0; c < 6; ++c) {
     u[(GXX + c) * n + i] = g[c].v;
-    du[(GXX + c) * n + i] = g[c].d;
     u[(AXX + c) * n + i] = A[c].v;
-    du[(AXX + c) * n + i] = A[c].d;
+#pragma unroll
+    for (int l = 0; l < ND; ++l) {
+      du[l * ds + (GXX + c) * n + i] = g[c].d[l];
+      du[l * ds + (AXX + c) * n + i] = A[c].d[l];
+    }
   }
 }
 
 // Hamiltonian constraint H = gamma^ij W^2 R_ij + 2/3 K^2 - A_ij A^ij
 // (reference bssn/constraints.py:271-303, W^2 R_ij of geometry.py:21-119),
 // with its tangent.
+template <int ND>
 __global__ void __launch_bounds__(128) k_hamiltonian_tangent(Ctx C, const double *__restrict__ us,
                                                              const double *__restrict__ dus, double *H,
                                                              double *dH) {
+  using Dual = DualN<ND>;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= C.ncell) return;
   const int z = (int)(i % C.ax[2].n);
   const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
   const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
   const int64_t n = C.ncell;
-  DualPointProvider pv;
+  const int64_t ds = (int64_t)NCOMP * n;
+  DualPointProvider<ND> pv;
   pv.setup(C, us, dus, x, y, z);
-  auto ld = [&](int c) { return Dual(us[c * n + i], dus[c * n + i]); };
+  auto ld = [&](int c) {
+    Dual r;
+    r.v = us[c * n + i];
+#pragma unroll
+    for (int l = 0; l < ND; ++l) r.d[l] = dus[l * ds + c * n + i];
+    return r;
+  };
   Dual g[6], A[6], ig[6];
   for (int c = 0; c < 6; ++c) {
     g[c] = ld(GXX + c);
@@ -318,7 +273,8 @@ __global__ void __launch_bounds__(128) k_hamiltonian_tangent(Ctx C, const double
   }
   const Dual h = contract(ig, R) + (2.0 / 3.0) * K * K - contract(A, Aup);
   H[i] = h.v;
-  dH[i] = h.d;
+#pragma unroll
+  for (int l = 0; l < ND; ++l) dH[l * n + i] = h.d[l];
 }
 
 int resolve_tangent(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
@@ -329,12 +285,44 @@ int resolve_tangent(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
   return BSSN_OK;
 }
 
-int launch_tangent_stage(const Ctx &C, const double *us, const double *dus, StageOut O, const TangentOut &T,
+// The tile kernels cover the FP64 configurations of the evolution's tile
+// path without the MAD blend: every tile (no closure shells), or interior
+// and face tiles when the closure faces sit on the tile grid.
+bool tangent_tiles(const Ctx &C) {
+  const bool mad = C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0;
+  return !mad && (!C.any_shell || tiles_aligned(C));
+}
+
+// The point kernels carry one direction; a stack of directions takes one
+// launch per direction, the first storing the value as well (the value
+// outputs of a stage may alias its inputs, e.g. ustar in stages 2 and 3, so
+// they are written exactly once).  A point-kernel instantiation carrying two
+// directions returned a wrong K right-hand side for whole workgroups on the
+// device (cause not identified), so none is built.
+int launch_tangent_points(const Ctx &C, int nd, const double *us, const double *dus, const StageOut &O,
+                          const TangentOut &T, hipStream_t st) {
+  const int64_t blocks = (C.ncell + 127) / 128, ds = (int64_t)NCOMP * C.ncell;
+  auto shift = [&](auto *q, int l) { return q ? q + l * ds : q; };
+  for (int l = 0; l < nd; ++l) {
+    const TangentOut Tl{shift(T.du0, l), shift(T.dustar, l), shift(T.dunext, l), shift(T.drhs, l), l > 0};
+    hipLaunchKernelGGL(k_stage_tangent<1>, dim3((unsigned)blocks), dim3(128), 0, st, C, us, dus + l * ds, O, Tl);
+    const int rc = check_hip(hipGetLastError(), "tangent stage kernel launch");
+    if (rc) return rc;
+  }
+  return BSSN_OK;
+}
+
+int launch_tangent_stage(const Ctx &C, int nd, const double *us, const double *dus, StageOut O, const TangentOut &T,
                          hipStream_t st) {
   stage_update(O, C.dt);
-  const int64_t blocks = (C.ncell + 127) / 128;
-  hipLaunchKernelGGL(k_stage_tangent, dim3((unsigned)blocks), dim3(128), 0, st, C, us, dus, O, T);
-  return check_hip(hipGetLastError(), "tangent stage kernel launch");
+  if (tangent_tiles(C))
+    return nd == 1 ? launch_tangent_tiles_1(C, us, dus, O, T, st) : launch_tangent_tiles_2(C, us, dus, O, T, st);
+  return launch_tangent_points(C, nd, us, dus, O, T, st);
+}
+
+int check_dirs(int ndir) {
+  if (ndir < 1 || ndir > MAX_DIRS) return fail(BSSN_ERR_INVALID, "ndir must be 1 or %d (got %d)", MAX_DIRS, ndir);
+  return BSSN_OK;
 }
 
 }  // namespace
@@ -344,44 +332,54 @@ using namespace bssn;
 
 extern "C" {
 
-int bssn_rhs_tangent(const bssn_params *p, const bssn_grid *grid, const double *state, const double *dstate,
-                     double *rhs, double *drhs, void *stream) {
+int bssn_rhs_tangent(const bssn_params *p, const bssn_grid *grid, int ndir, const double *state,
+                     const double *dstate, double *rhs, double *drhs, void *stream) {
   Ctx C;
   int rc = resolve_tangent(p, grid, &C);
-  if (rc) return rc;
+  if (rc || (rc = check_dirs(ndir))) return rc;
   if (!state || !dstate || !rhs || !drhs) return fail(BSSN_ERR_INVALID, "null buffer");
   StageOut O{nullptr, nullptr, nullptr, rhs, 0, 0, nullptr, nullptr};
   TangentOut T{nullptr, nullptr, nullptr, drhs};
-  return launch_tangent_stage(C, state, dstate, O, T, (hipStream_t)stream);
+  return launch_tangent_stage(C, ndir, state, dstate, O, T, (hipStream_t)stream);
 }
 
-int bssn_evolve_tangent(const bssn_params *p, const bssn_grid *grid, int nsteps, double *state, double *dstate,
-                        double *work, void *stream) {
+int64_t bssn_tangent_work_size(const bssn_grid *grid, int ndir) {
+  if (!grid || grid->nx <= 0 || grid->ny <= 0 || grid->nz <= 0) return fail(BSSN_ERR_INVALID, "invalid grid");
+  if (ndir < 1 || ndir > MAX_DIRS) return fail(BSSN_ERR_INVALID, "ndir must be 1 or %d (got %d)", MAX_DIRS, ndir);
+  return 3 * (int64_t)(1 + ndir) * NCOMP * (int64_t)grid->nx * grid->ny * grid->nz;
+}
+
+int bssn_evolve_tangent(const bssn_params *p, const bssn_grid *grid, int nsteps, int ndir, double *state,
+                        double *dstate, double *work, void *stream) {
   Ctx C;
   int rc = resolve_tangent(p, grid, &C);
-  if (rc) return rc;
+  if (rc || (rc = check_dirs(ndir))) return rc;
   if (nsteps < 0) return fail(BSSN_ERR_INVALID, "nsteps must be non-negative");
   if (!state || !dstate || !work) return fail(BSSN_ERR_INVALID, "null buffer");
   if (nsteps == 0) return BSSN_OK;
   hipStream_t st = (hipStream_t)stream;
-  const int64_t n = C.ncell * NCOMP;
+  const int64_t n = C.ncell * NCOMP, dn = n * ndir;
   double *ustar = work, *ua = work + n, *ub = work + 2 * n;
-  double *dustar = work + 3 * n, *dua = work + 4 * n, *dub = work + 5 * n;
+  double *dustar = work + 3 * n, *dua = dustar + dn, *dub = dua + dn;
   const int64_t pblocks = (C.ncell + 255) / 256;
-  hipLaunchKernelGGL(k_project_tangent, dim3((unsigned)pblocks), dim3(256), 0, st, C.ncell, state, dstate);
+  if (ndir == 1)
+    hipLaunchKernelGGL(k_project_tangent<1>, dim3((unsigned)pblocks), dim3(256), 0, st, C.ncell, state, dstate);
+  else
+    hipLaunchKernelGGL(k_project_tangent<2>, dim3((unsigned)pblocks), dim3(256), 0, st, C.ncell, state, dstate);
   if ((rc = check_hip(hipGetLastError(), "tangent projection launch"))) return rc;
   for (int s = 0; s < nsteps; ++s) {
     const int last = s == nsteps - 1;
-    if ((rc = launch_tangent_stage(C, state, dstate, StageOut{state, ustar, ua, nullptr, 1, 0, nullptr, nullptr},
+    if ((rc = launch_tangent_stage(C, ndir, state, dstate,
+                                   StageOut{state, ustar, ua, nullptr, 1, 0, nullptr, nullptr},
                                    TangentOut{dstate, dustar, dua, nullptr}, st)))
       return rc;
-    if ((rc = launch_tangent_stage(C, ua, dua, StageOut{state, ustar, ub, nullptr, 2, 0, nullptr, nullptr},
+    if ((rc = launch_tangent_stage(C, ndir, ua, dua, StageOut{state, ustar, ub, nullptr, 2, 0, nullptr, nullptr},
                                    TangentOut{dstate, dustar, dub, nullptr}, st)))
       return rc;
-    if ((rc = launch_tangent_stage(C, ub, dub, StageOut{state, ustar, ua, nullptr, 3, 0, nullptr, nullptr},
+    if ((rc = launch_tangent_stage(C, ndir, ub, dub, StageOut{state, ustar, ua, nullptr, 3, 0, nullptr, nullptr},
                                    TangentOut{dstate, dustar, dua, nullptr}, st)))
       return rc;
-    if ((rc = launch_tangent_stage(C, ua, dua,
+    if ((rc = launch_tangent_stage(C, ndir, ua, dua,
                                    StageOut{nullptr, ustar, state, nullptr, 4, last ? 0 : 1, nullptr, nullptr},
                                    TangentOut{nullptr, dustar, dstate, nullptr}, st)))
       return rc;
@@ -389,18 +387,21 @@ int bssn_evolve_tangent(const bssn_params *p, const bssn_grid *grid, int nsteps,
   return BSSN_OK;
 }
 
-int bssn_hamiltonian_tangent(const bssn_params *p, const bssn_grid *grid, const double *state,
+int bssn_hamiltonian_tangent(const bssn_params *p, const bssn_grid *grid, int ndir, const double *state,
                              const double *dstate, double *H, double *dH, void *stream) {
   Ctx C;
   bssn_params q = p ? *p : bssn_params{};
   q.kappa = 0.0;  // the constraint does not depend on kappa
   int rc = resolve_tangent(&q, grid, &C);
-  if (rc) return rc;
+  if (rc || (rc = check_dirs(ndir))) return rc;
   if (!state || !dstate || !H || !dH) return fail(BSSN_ERR_INVALID, "null buffer");
   const int64_t blocks = (C.ncell + 127) / 128;
-  hipLaunchKernelGGL(k_hamiltonian_tangent, dim3((unsigned)blocks), dim3(128), 0, (hipStream_t)stream, C, state,
-                     dstate, H, dH);
-  return check_hip(hipGetLastError(), "tangent Hamiltonian kernel launch");
+  for (int l = 0; l < ndir; ++l) {  // one direction per launch, as the point stage kernel
+    hipLaunchKernelGGL(k_hamiltonian_tangent<1>, dim3((unsigned)blocks), dim3(128), 0, (hipStream_t)stream, C,
+                       state, dstate + l * NCOMP * C.ncell, H, dH + l * C.ncell);
+    if ((rc = check_hip(hipGetLastError(), "tangent Hamiltonian kernel launch"))) return rc;
+  }
+  return BSSN_OK;
 }
 
 }  // extern "C"

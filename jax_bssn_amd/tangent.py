@@ -4,10 +4,15 @@ L2 Hamiltonian constraint with respect to initial-data parameters.
 
 The reference differentiates its JAX evolution with autodiff.  Here the
 per-point RHS, the RK4 stage update and the projection are instantiated with
-dual numbers (value, derivative along one direction), so one tangent run
-gives the exact derivative of the whole evolution along one initial-data
-direction; p parameters take p tangent runs and no checkpoints (nothing is
-replayed backwards).  kappa = 0 and vacuum only.
+dual numbers (value and derivatives along one or two directions), so one
+tangent run gives the exact derivatives of the whole evolution along two
+initial-data directions at once; p parameters take ceil(p / 2) tangent runs
+and no checkpoints (nothing is replayed backwards).  kappa = 0 and vacuum
+only.
+
+A tangent argument ``dstate`` is either one direction (the shape of the
+state) or a stack of up to two directions (``(ndir,) + state.shape``);
+results keep the form they were given in.
 """
 
 import ctypes
@@ -21,42 +26,64 @@ from .time_evolve import _ptr, stream_handle
 from .variables import NCOMP, device
 
 
+MAX_DIRS = 2
+
+
 def _prep(t):
     return t.to(device(), torch.float64).contiguous().clone()
 
 
+def _dirs(state, dstate):
+    """(stacked (ndir, 21, ...) tangent tensor, whether the input was one
+    direction)."""
+    single = dstate.dim() == state.dim()
+    d = _prep(dstate.unsqueeze(0) if single else dstate)
+    if tuple(d.shape[1:]) != tuple(state.shape) or not 1 <= d.shape[0] <= MAX_DIRS:
+        raise ValueError(f"tangent shape {tuple(dstate.shape)} does not fit state {tuple(state.shape)} "
+                         f"with 1..{MAX_DIRS} directions")
+    return d, single
+
+
 def rhs_tangent(state, dstate, params):
     """(RHS(state), dRHS(state)[dstate]) of packed device states."""
-    state, dstate = _prep(state), _prep(dstate)
-    rhs, drhs = torch.empty_like(state), torch.empty_like(state)
+    state = _prep(state)
+    d, single = _dirs(state, dstate)
+    rhs, drhs = torch.empty_like(state), torch.empty_like(d)
     _lib.check(_lib.load().bssn_rhs_tangent(ctypes.byref(_lib.c_params(params)),
-                                            ctypes.byref(_lib.c_grid(state.shape[1:])), _ptr(state), _ptr(dstate),
-                                            _ptr(rhs), _ptr(drhs), stream_handle()))
-    return rhs, drhs
+                                            ctypes.byref(_lib.c_grid(state.shape[1:])), int(d.shape[0]),
+                                            _ptr(state), _ptr(d), _ptr(rhs), _ptr(drhs), stream_handle()))
+    return rhs, (drhs[0] if single else drhs)
 
 
 def evolve_tangent(state, dstate, params, nsteps):
-    """``nsteps`` RK4 steps of a packed state with its tangent carried along."""
-    state, dstate = _prep(state), _prep(dstate)
-    work = torch.empty(6 * state.numel(), dtype=torch.float64, device=state.device)
-    _lib.check(_lib.load().bssn_evolve_tangent(ctypes.byref(_lib.c_params(params)),
-                                               ctypes.byref(_lib.c_grid(state.shape[1:])), int(nsteps),
-                                               _ptr(state), _ptr(dstate), _ptr(work), stream_handle()))
-    return state, dstate
+    """``nsteps`` RK4 steps of a packed state with its tangents carried along."""
+    state = _prep(state)
+    d, single = _dirs(state, dstate)
+    lib = _lib.load()
+    grid = _lib.c_grid(state.shape[1:])
+    size = int(lib.bssn_tangent_work_size(ctypes.byref(grid), int(d.shape[0])))
+    if size < 0:
+        _lib.check(size)
+    work = torch.empty(size, dtype=torch.float64, device=state.device)
+    _lib.check(lib.bssn_evolve_tangent(ctypes.byref(_lib.c_params(params)), ctypes.byref(grid), int(nsteps),
+                                       int(d.shape[0]), _ptr(state), _ptr(d), _ptr(work), stream_handle()))
+    return state, (d[0] if single else d)
 
 
 def hamiltonian_l2_tangent(state, dstate, params):
     """(L, dL) with L = sqrt(mean H^2) over all points and dL its derivative
-    along dstate (mean(H dH) / L)."""
+    along each direction of dstate (mean(H dH) / L); dL is a float for one
+    direction and a list for a stack."""
+    d, single = _dirs(state, dstate)
     shape = tuple(state.shape[1:])
     H = torch.empty(shape, dtype=torch.float64, device=state.device)
-    dH = torch.empty_like(H)
+    dH = torch.empty((d.shape[0],) + shape, dtype=torch.float64, device=state.device)
     _lib.check(_lib.load().bssn_hamiltonian_tangent(ctypes.byref(_lib.c_params(params)),
-                                                    ctypes.byref(_lib.c_grid(shape)), _ptr(state), _ptr(dstate),
-                                                    _ptr(H), _ptr(dH), stream_handle()))
+                                                    ctypes.byref(_lib.c_grid(shape)), int(d.shape[0]),
+                                                    _ptr(state), _ptr(d), _ptr(H), _ptr(dH), stream_handle()))
     L = torch.sqrt(torch.mean(H * H))
-    dL = torch.mean(H * dH) / L
-    return float(L), float(dL)
+    dL = [float(torch.mean(H * dH[l]) / L) for l in range(d.shape[0])]
+    return float(L), (dL[0] if single else dL)
 
 
 class HamiltonianL2AfterSteps(torch.autograd.Function):
@@ -66,18 +93,19 @@ class HamiltonianL2AfterSteps(torch.autograd.Function):
     ``initial(theta)`` (theta a NumPy vector) returns the packed initial
     state and, for every component of theta, the packed derivative of the
     initial state with respect to it.  The forward pass runs one tangent
-    evolution per component; backward multiplies the stored gradient by the
-    incoming gradient.
+    evolution per pair of components; backward multiplies the stored
+    gradient by the incoming gradient.
     """
 
     @staticmethod
     def forward(ctx, theta, initial, params, nsteps):
         u0, tangents = initial(theta.detach().cpu().numpy())
         value, grads = None, []
-        for du0 in tangents:
+        for k in range(0, len(tangents), MAX_DIRS):
+            du0 = torch.stack(tangents[k:k + MAX_DIRS])
             u, du = evolve_tangent(u0, du0, params, nsteps)
             value, dL = hamiltonian_l2_tangent(u, du, params)
-            grads.append(dL)
+            grads.extend(dL)
         ctx.save_for_backward(torch.tensor(grads, dtype=theta.dtype))
         return torch.tensor(value, dtype=theta.dtype)
 

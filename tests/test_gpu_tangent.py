@@ -1,7 +1,11 @@
 """GPU checks of the forward-mode differentiation (dual-number kernels).
 
 * the value part of the tangent kernels equals the ordinary kernels
-  (1e-10 RHS, the RHS bar; 1e-12 per step);
+  (1e-10 RHS, the RHS bar; 1e-12 per step), on the three kernel paths:
+  periodic tile kernel, tile kernels with Sommerfeld face tiles, and the
+  point kernel (Sommerfeld faces off the tile grid);
+* a run carrying two directions equals the two single-direction runs
+  (1e-13: the same arithmetic per direction);
 * the tangent RHS and the tangent evolution agree with central finite
   differences of the ordinary kernels (step h = 1e-4 relative; the
   truncation error is O(h^2), so agreement to 1e-6 relative per field);
@@ -46,8 +50,10 @@ def _cases(J):
     v2 = smooth_state((16, 17, 18), seed=7)
     p2 = J.BSSNParameters(eta=2.0, g=0.75, nu=0.1, dx=0.1, dt=0.025, zero_shift=0, gauge=1, xl_bc=1, xr_bc=1,
                           yl_bc=1, yr_bc=1, zl_bc=1, zr_bc=1, x_min=-0.8, y_min=-0.8, z_min=-0.9)
+    v3 = smooth_state((16, 16, 24), seed=9)
+    p3 = p2._replace(dx=0.1, dt=0.025, z_min=-1.2)
     out = []
-    for vv, pp, shape, seed in ((v, p, (16, 16, 16), 15), (v2, p2, (16, 17, 18), 17)):
+    for vv, pp, shape, seed in ((v, p, (16, 16, 16), 15), (v2, p2, (16, 17, 18), 17), (v3, p3, (16, 16, 24), 19)):
         u = pack(vv)
         # a smooth direction (the difference of two smooth states), so that
         # the finite differences stay in their asymptotic range
@@ -81,6 +87,25 @@ def test_evolve_tangent_matches_finite_differences(J):
         err = _field_rel(ds, fd)
         print("evolve tangent vs FD", f"{err:.1e}", flush=True)
         assert err <= 1e-6
+
+
+def test_two_directions_equal_single_runs(J):
+    from jax_bssn_amd.tangent import evolve_tangent, rhs_tangent
+    for u, du, p in _cases(J):
+        du2 = torch.roll(du, 3, dims=1) * 0.7 + 0.01 * du.flip(1)
+        both = torch.stack([du, du2])
+        r, dr = rhs_tangent(u, both, p)
+        for l, d in enumerate((du, du2)):
+            r1, dr1 = rhs_tangent(u, d, p)
+            assert torch.equal(r, r1) or _field_rel(r, r1) <= 1e-13
+            assert _field_rel(dr[l], dr1) <= 1e-13
+        s, ds = evolve_tangent(u, both, p, 2)
+        for l, d in enumerate((du, du2)):
+            s1, ds1 = evolve_tangent(u, d, p, 2)
+            assert _field_rel(s, s1) <= 1e-13
+            err = _field_rel(ds[l], ds1)
+            print("two directions vs one", l, f"{err:.1e}", flush=True)
+            assert err <= 1e-13
 
 
 def _puncture_setup(J, n=24, L=4.0):
