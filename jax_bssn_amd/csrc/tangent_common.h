@@ -55,21 +55,25 @@ struct DualTileProvider {
     dstride = (int64_t)NCOMP * C.ncell;
   }
   // Stage visit v: every box loaded between two barriers (the first waits
-  // until every lane is done with visit v - 1).
+  // until every lane is done with visit v - 1).  The loads of all boxes are
+  // issued before the first LDS store, so their latencies overlap.
   __device__ __forceinline__ void stage(int v) {
     const int64_t off = (int64_t)visit_comp(v) * b.ncell;
-    if (v > 0) __syncthreads();
+    double t[ND + 1][NSLOT];
 #pragma unroll
     for (int l = 0; l <= ND; ++l) {
       const BSSN_GLOBAL double *f = (l == 0 ? src : dsrc + (l - 1) * dstride) + off;
-      BSSN_LDS double *buf = lds + l * NBOX;
-      double t[NSLOT];
 #pragma unroll
-      for (int k = 0; k < NSLOT; ++k) t[k] = f[b.goff[k]];
+      for (int k = 0; k < NSLOT; ++k) t[l][k] = f[b.goff[k]];
+    }
+    if (v > 0) __syncthreads();
+#pragma unroll
+    for (int l = 0; l <= ND; ++l) {
+      BSSN_LDS double *buf = lds + l * NBOX;
 #pragma unroll
       for (int k = 0; k < NSLOT; ++k) {
         const int e = b.tid + k * NT;
-        if (k < NSLOT - 1 || e < NBOX) buf[e] = t[k];
+        if (k < NSLOT - 1 || e < NBOX) buf[e] = t[l][k];
       }
     }
     __syncthreads();
