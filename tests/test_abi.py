@@ -102,3 +102,24 @@ def test_work_sizes(lib):
     assert lib.bssn_work_size(pk, g, 1) == 3 * 21 * n + 30 * n
     with pytest.raises(ValueError, match="kappa"):
         _lib.check(lib.bssn_rhs(pk, g, 0, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), None, None))
+
+
+def test_tangent_argument_validation(lib):
+    from jax_bssn_amd.variables import BSSNParameters
+    g = ctypes.byref(_lib.c_grid((8, 9, 10)))
+    n = 8 * 9 * 10
+    assert lib.bssn_tangent_work_size(g, 1) == 3 * 2 * 21 * n
+    assert lib.bssn_tangent_work_size(g, 2) == 3 * 3 * 21 * n
+    with pytest.raises(ValueError, match="ndir"):
+        _lib.check(lib.bssn_tangent_work_size(g, 3))
+    p = ctypes.byref(_lib.c_params(BSSNParameters(dx=0.1)))
+    buf = ctypes.c_void_p(0x1000)
+    with pytest.raises(ValueError, match="ndir"):
+        _lib.check(lib.bssn_rhs_tangent(p, g, 0, buf, buf, buf, buf, None))
+    with pytest.raises(ValueError, match="ndir"):
+        _lib.check(lib.bssn_evolve_tangent(p, g, 1, 3, buf, buf, buf, None))
+    with pytest.raises(ValueError, match="null"):
+        _lib.check(lib.bssn_hamiltonian_tangent(p, g, 2, buf, None, buf, buf, None))
+    pk = ctypes.byref(_lib.c_params(BSSNParameters(dx=0.1, kappa=0.02)))
+    with pytest.raises(NotImplementedError, match="kappa"):
+        _lib.check(lib.bssn_evolve_tangent(pk, g, 1, 1, buf, buf, buf, None))
