@@ -41,10 +41,10 @@ __device__ __forceinline__ void load_point(const Ctx &C, const double *__restric
 
 // ------------------------------------------------------------ tile kernel
 
-template <typename T, int TX, int TY, int TZ, bool MAD, bool CL>
+template <typename T, int TX, int TY, int TZ, bool MAD, bool CL, int CLA = 7>
 __global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
     k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, TileSet S) {
-  using P = TileProvider<T, TX, TY, TZ, MAD, CL>;
+  using P = TileProvider<T, TX, TY, TZ, MAD, CL, CLA>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   int bx, by, bz, slab;
@@ -133,6 +133,15 @@ inline bool tiles_aligned(const Ctx &C) {
   return true;
 }
 
+template <typename T, bool MAD, int CLA>
+inline int launch_face_tiles(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int kind) {
+  const TileSet S = tile_set(C, kind, TX, TY, TZ);
+  if (S.count <= 0) return BSSN_OK;
+  hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true, CLA>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
+                     O, S);
+  return check_hip(hipGetLastError(), "tile kernel launch");
+}
+
 template <typename T, bool MAD, bool CL>
 inline int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
   int rc = BSSN_OK;
@@ -146,15 +155,19 @@ inline int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hip
       if (rc) return rc;
     }
   }
-  // closure-capable kernel on the face tiles
+  // closure-capable kernels on the face tiles: one per slab whose tiles
+  // have closures along that axis only, one for edges and corners (with
+  // MAD every face tile takes the general kernel)
   if (CL && (parts & (PART_TILES | PART_CLOSURE))) {
-    const TileSet S = tile_set(C, 2, TX, TY, TZ);
-    if (S.count > 0) {
-      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
-                         O, S);
-      rc = check_hip(hipGetLastError(), "tile kernel launch");
-      if (rc) return rc;
+    if constexpr (MAD) {
+      rc = launch_face_tiles<T, MAD, 7>(C, us, O, st, 2);
+    } else {
+      if (!rc) rc = launch_face_tiles<T, MAD, 1>(C, us, O, st, 3);
+      if (!rc) rc = launch_face_tiles<T, MAD, 2>(C, us, O, st, 4);
+      if (!rc) rc = launch_face_tiles<T, MAD, 4>(C, us, O, st, 5);
+      if (!rc) rc = launch_face_tiles<T, MAD, 7>(C, us, O, st, 6);
     }
+    if (rc) return rc;
   }
   if ((parts & PART_SHELL) && C.any_shell && !CL) {
     Slabs S = make_slabs(C);

@@ -237,9 +237,13 @@ __host__ __device__ constexpr int tile_halo(int t, bool mad, bool cl) {
 static_assert(tile_halo(2, false, true) == 6 && tile_halo(4, false, true) == 4 && tile_halo(8, false, true) == 3,
               "closure halo");
 
-template <typename T, int TX, int TY, int TZ, bool MAD, bool CL>
+// CL: the tile may hold closure points; CLA: the axes (bit a) along which
+// they may lie (a face tile of one slab has closures along one axis only,
+// which keeps the other axes' stencils compact and their halos narrow).
+template <typename T, int TX, int TY, int TZ, bool MAD, bool CL, int CLA = 7>
 struct TileProvider {
-  static constexpr int HX = tile_halo(TX, MAD, CL), HY = tile_halo(TY, MAD, CL), HZ = tile_halo(TZ, MAD, CL);
+  static constexpr bool CLX = CL && (CLA & 1), CLY = CL && (CLA & 2), CLZ = CL && (CLA & 4);
+  static constexpr int HX = tile_halo(TX, MAD, CLX), HY = tile_halo(TY, MAD, CLY), HZ = tile_halo(TZ, MAD, CLZ);
   static constexpr int NT = TX * TY * TZ;
   static constexpr int LX = TX + 2 * HX, LY = TY + 2 * HY, LZ = TZ + 2 * HZ;
   static constexpr int SX = LY * LZ, SY = LZ;
@@ -306,8 +310,9 @@ struct TileProvider {
       red[a] = C.ax[a].reduced;
       ax[a] = C.ax[a];
       pos[a] = o[a] + l[a];
-      clz[a] = CL && ((C.ax[a].lbc && o[a] < C.ax[a].start + 3 && o[a] + t[a] > C.ax[a].start) ||
-                      (C.ax[a].rbc && o[a] + t[a] > C.ax[a].end - 3 && o[a] < C.ax[a].end));
+      clz[a] = CL && ((CLA >> a) & 1) &&
+               ((C.ax[a].lbc && o[a] < C.ax[a].start + 3 && o[a] + t[a] > C.ax[a].start) ||
+                (C.ax[a].rbc && o[a] + t[a] > C.ax[a].end - 3 && o[a] < C.ax[a].end));
     }
     isface = false;
     if constexpr (CL) {
@@ -363,7 +368,7 @@ struct TileProvider {
     if constexpr (!CL) {
       return false;
     } else {
-      if (!clz[a]) return false;
+      if (!((CLA >> a) & 1) || !clz[a]) return false;
       const int lo = pos[a] - ax[a].start, hi = ax[a].end - 1 - pos[a];
       return (ax[a].lbc && lo >= 0 && lo < w) || (ax[a].rbc && hi >= 0 && hi < w);
     }

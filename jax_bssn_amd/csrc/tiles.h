@@ -16,10 +16,16 @@ namespace bssn {
 // are then the product of the three per-axis complements, and the closure
 // tiles the rest: the x slab (bx in the ranges), the y slab (bx outside,
 // by inside) and the z slab (bx, by outside, bz inside).
+//
+// The closure tiles split further by the axes their closures lie along:
+// kinds 3, 4, 5 the tiles with closures along x, y, z only (one slab's
+// faces), kind 6 the tiles with closures along two or three axes (edges and
+// corners), enumerated as x-and-y ranges (any z), x-and-z ranges (y
+// outside) and y-and-z ranges (x outside).
 struct TileSet {
   int n[3];                       // tiles per axis
   int l0[3], l1[3], r0[3], r1[3];  // closure-touching tile ranges per axis
-  int kind;                       // 0 every tile, 1 tiles without closures, 2 with
+  int kind;                       // 0 every tile, 1 tiles without closures, 2 with, 3..6 as above
   int count;                      // tiles launched (one workgroup each)
 };
 
@@ -53,7 +59,36 @@ BSSN_HD __forceinline__ void tile_of(const TileSet &S, int k, int &bx, int &by, 
     bx = compl_pick(S, 0, k / (cz * cy));
     return;
   }
-  const int sx = range_size(S, 0), sy = range_size(S, 1);
+  const int sx = range_size(S, 0), sy = range_size(S, 1), sz = range_size(S, 2);
+  if (S.kind >= 3 && S.kind <= 5) {
+    const int a = S.kind - 3;  // the closure axis
+    if (slab) *slab = a;
+    const int ez = a == 2 ? sz : cz, ey = a == 1 ? sy : cy;
+    const int jz = k % ez, jy = (k / ez) % ey, jx = k / (ez * ey);
+    bz = a == 2 ? range_pick(S, 2, jz) : compl_pick(S, 2, jz);
+    by = a == 1 ? range_pick(S, 1, jy) : compl_pick(S, 1, jy);
+    bx = a == 0 ? range_pick(S, 0, jx) : compl_pick(S, 0, jx);
+    return;
+  }
+  if (S.kind == 6) {
+    const int nxy = sx * sy * S.n[2], nxz = sx * cy * sz;
+    if (k < nxy) {
+      bz = k % S.n[2];
+      by = range_pick(S, 1, (k / S.n[2]) % sy);
+      bx = range_pick(S, 0, k / (S.n[2] * sy));
+    } else if ((k -= nxy) < nxz) {
+      bz = range_pick(S, 2, k % sz);
+      by = compl_pick(S, 1, (k / sz) % cy);
+      bx = range_pick(S, 0, k / (sz * cy));
+    } else {
+      k -= nxz;
+      if (slab) *slab = 1;
+      bz = range_pick(S, 2, k % sz);
+      by = range_pick(S, 1, (k / sz) % sy);
+      bx = compl_pick(S, 0, k / (sz * sy));
+    }
+    return;
+  }
   const int nx_slab = sx * S.n[1] * S.n[2], ny_slab = cx * sy * S.n[2];
   if (k < nx_slab) {
     bz = k % S.n[2];
@@ -67,7 +102,6 @@ BSSN_HD __forceinline__ void tile_of(const TileSet &S, int k, int &bx, int &by, 
   } else {
     if (slab) *slab = 2;
     k -= ny_slab;
-    const int sz = range_size(S, 2);
     bz = range_pick(S, 2, k % sz);
     by = compl_pick(S, 1, (k / sz) % cy);
     bx = compl_pick(S, 0, k / (sz * cy));
@@ -93,7 +127,7 @@ BSSN_HD inline bool tile_has_closure(const Ctx &C, int ox, int oy, int oz, int t
 inline TileSet tile_set(const Ctx &C, int kind, int tx, int ty, int tz) {
   TileSet S;
   const int t[3] = {tx, ty, tz};
-  int total = 1, lean = 1;
+  int total = 1, lean = 1, ns[3], nc[3];
   for (int a = 0; a < 3; ++a) {
     const Axis &A = C.ax[a];
     const int n = (A.n + t[a] - 1) / t[a];
@@ -117,10 +151,20 @@ inline TileSet tile_set(const Ctx &C, int kind, int tx, int ty, int tz) {
     }
     S.l0[a] = l0, S.l1[a] = l1, S.r0[a] = r0, S.r1[a] = r1;
     total *= n;
-    lean *= n - ((l1 - l0) + (r1 - r0));
+    ns[a] = (l1 - l0) + (r1 - r0);
+    nc[a] = n - ns[a];
+    lean *= nc[a];
   }
   S.kind = kind;
-  S.count = kind == 0 ? total : (kind == 1 ? lean : total - lean);
+  switch (kind) {
+    case 0: S.count = total; break;
+    case 1: S.count = lean; break;
+    case 2: S.count = total - lean; break;
+    case 3: S.count = ns[0] * nc[1] * nc[2]; break;
+    case 4: S.count = nc[0] * ns[1] * nc[2]; break;
+    case 5: S.count = nc[0] * nc[1] * ns[2]; break;
+    default: S.count = ns[0] * ns[1] * S.n[2] + ns[0] * nc[1] * ns[2] + nc[0] * ns[1] * ns[2]; break;
+  }
   return S;
 }
 

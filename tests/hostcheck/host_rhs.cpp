@@ -67,8 +67,9 @@ extern "C" int host_stage(const bssn_params *p, const bssn_grid *g, int stage, c
 }
 
 // Tile sets against the brute-force predicate: every tile of kind 1 (2)
-// enumerated exactly once iff it has no (has) closure points, and the point
-// count of the two sets adds up to the grid.  Returns the number of
+// enumerated exactly once iff it has no (has) closure points, the point
+// count of the two sets adds up to the grid, and kinds 3..6 enumerate the
+// closure tiles once each, split by the axes of their closures.  Returns the number of
 // violations (or a negative status).
 extern "C" int host_tile_check(const bssn_params *p, const bssn_grid *g, int tx, int ty, int tz, char *err,
                                int errlen) {
@@ -97,6 +98,43 @@ extern "C" int host_tile_check(const bssn_params *p, const bssn_grid *g, int tx,
         }
     pts += tile_set_points(C, S, tx, ty, tz);
   }
+  // kinds 3..6 partition the closure tiles by the axes their closures lie
+  // along: 3, 4, 5 exactly x, y, z; 6 two or three axes
+  const int t[3] = {tx, ty, tz};
+  auto axes_of = [&](int bx, int by, int bz) {
+    const int o[3] = {bx * tx, by * ty, bz * tz};
+    int mask = 0;
+    for (int a = 0; a < 3; ++a) {
+      const Axis &A = C.ax[a];
+      if ((A.lbc && o[a] < A.start + 3 && o[a] + t[a] > A.start) || (A.rbc && o[a] + t[a] > A.end - 3 && o[a] < A.end))
+        mask |= 1 << a;
+    }
+    return mask;
+  };
+  const TileSet S2 = tile_set(C, 2, tx, ty, tz);
+  std::vector<int> seen((size_t)S2.n[0] * S2.n[1] * S2.n[2], 0);
+  int count = 0;
+  for (int kind = 3; kind <= 6; ++kind) {
+    const TileSet S = tile_set(C, kind, tx, ty, tz);
+    count += S.count;
+    for (int k = 0; k < S.count; ++k) {
+      int bx, by, bz, slab;
+      tile_of(S, k, bx, by, bz, &slab);
+      if (bx < 0 || bx >= S.n[0] || by < 0 || by >= S.n[1] || bz < 0 || bz >= S.n[2]) {
+        ++bad;
+        continue;
+      }
+      seen[((size_t)bx * S.n[1] + by) * S.n[2] + bz] += 1;
+      const int m = axes_of(bx, by, bz);
+      const bool ok = kind == 6 ? (m != 0 && (m & (m - 1)) != 0) : (m == 1 << (kind - 3) && slab == kind - 3);
+      if (!ok) ++bad;
+    }
+  }
+  if (count != S2.count) ++bad;
+  for (int bx = 0; bx < S2.n[0]; ++bx)
+    for (int by = 0; by < S2.n[1]; ++by)
+      for (int bz = 0; bz < S2.n[2]; ++bz)
+        if (seen[((size_t)bx * S2.n[1] + by) * S2.n[2] + bz] != (axes_of(bx, by, bz) ? 1 : 0)) ++bad;
   return bad + (pts != C.ncell ? 1 : 0);
 }
 
