@@ -63,10 +63,12 @@ def time_steps(J, state, params, steps, warmup, precision):
     return e0.elapsed_time(e1) / 1e3, ev
 
 
-KERNEL_NAMES = {("f64", 4): "k_stage_tile<double, 4, 8, 8, false, false>",
-                ("f64", 8): "k_stage_tile<double, 4, 8, 8, false, true>",
-                ("f32rhs", 4): "k_stage_tile<float, 4, 8, 8, false, false>",
-                ("f32rhs", 8): "k_stage_tile<float, 4, 8, 8, false, true>"}
+# the interior tile kernel, and the face-tile kernels (one per slab with
+# closures along its axis, CLA = 1, 2, 4, and the edges and corners, CLA = 7)
+KERNEL_NAMES = {("f64", 4): "k_stage_tile<double, 4, 8, 8, false, false, 7>",
+                ("f64", 8): "k_stage_tile<double, 4, 8, 8, false, true, CLA> (CLA = 1, 2, 4, 7)",
+                ("f32rhs", 4): "k_stage_tile<float, 4, 8, 8, false, false, 7>",
+                ("f32rhs", 8): "k_stage_tile<float, 4, 8, 8, false, true, CLA> (CLA = 1, 2, 4, 7)"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_tile_puncture256_traffic.md")
 
 
