@@ -63,16 +63,40 @@ __device__ __forceinline__ void axis_stencil(const ProlongGeom &G, int a, int i,
   }
 }
 
+// Points of the guard shell: the two x slabs (full y-z planes), the two y
+// slabs inside the active x range, the two z slabs inside the active x-y
+// range; consecutive threads run along z within each, so every wave is full.
+__host__ __device__ __forceinline__ int64_t shell_points(const ProlongGeom &G) {
+  const int64_t g2 = 2 * G.g;
+  return g2 * G.child.n[1] * G.child.n[2] + (int64_t)G.active[0] * g2 * G.child.n[2] +
+         (int64_t)G.active[0] * G.active[1] * g2;
+}
+
 __global__ void __launch_bounds__(256) k_fmr_prolong(ProlongGeom G, const double *__restrict__ parent,
                                                      double *__restrict__ child) {
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= G.child.ncell) return;
-  const int k = (int)(id % G.child.n[2]);
-  const int j = (int)((id / G.child.n[2]) % G.child.n[1]);
-  const int i = (int)(id / ((int64_t)G.child.n[2] * G.child.n[1]));
-  const bool active = i >= G.g && i < G.g + G.active[0] && j >= G.g && j < G.g + G.active[1] && k >= G.g &&
-                      k < G.g + G.active[2];
-  if (active) return;
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= shell_points(G)) return;
+  const int g = G.g, g2 = 2 * G.g, NY = G.child.n[1], NZ = G.child.n[2];
+  const int64_t nxs = (int64_t)g2 * NY * NZ, nys = (int64_t)G.active[0] * g2 * NZ;
+  int i, j, k;
+  if (r < nxs) {
+    k = (int)(r % NZ);
+    j = (int)((r / NZ) % NY);
+    const int t = (int)(r / ((int64_t)NZ * NY));
+    i = t < g ? t : t + G.active[0];
+  } else if ((r -= nxs) < nys) {
+    k = (int)(r % NZ);
+    const int t = (int)((r / NZ) % g2);
+    j = t < g ? t : t + G.active[1];
+    i = g + (int)(r / ((int64_t)NZ * g2));
+  } else {
+    r -= nys;
+    const int t = (int)(r % g2);
+    k = t < g ? t : t + G.active[2];
+    j = g + (int)((r / g2) % G.active[1]);
+    i = g + (int)(r / ((int64_t)g2 * G.active[1]));
+  }
+  const int64_t id = ((int64_t)i * NY + j) * NZ + k;
   int nx[6], ny[6], nz[6];
   double wx[6], wy[6], wz[6];
   axis_stencil(G, 0, i, nx, wx);
@@ -177,7 +201,8 @@ int bssn_fmr_prolong(const bssn_grid *parent_grid, int32_t parent_ghost, const i
         return fail(BSSN_ERR_INVALID, "child prolongation stencil exceeds parent allocation on axis %d", a);
     }
   }
-  const int64_t blocks = (G.child.ncell + 255) / 256;
+  const int64_t blocks = (shell_points(G) + 255) / 256;
+  if (blocks == 0) return BSSN_OK;
   hipLaunchKernelGGL(k_fmr_prolong, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, G, parent, child);
   return check_hip(hipGetLastError(), "prolongation kernel launch");
 }
