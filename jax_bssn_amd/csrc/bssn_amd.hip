@@ -246,6 +246,43 @@ __global__ void __launch_bounds__(256) k_kappa(Ctx C, const double *__restrict__
   kappa_point(C, us, D, matter, (int64_t)x * C.ax[0].stride + (int64_t)y * C.ax[1].stride + z, kap);
 }
 
+// The kappa term on tiles: the components of the state, the derived fields
+// and the matter sources are staged one at a time in one LDS box (the tile
+// stencils, closure rows on face tiles), instead of global-memory taps.
+template <int TX_, int TY_, int TZ_, bool CL>
+struct KappaTaps {
+  TileProvider<double, TX_, TY_, TZ_, false, CL> *tp;
+  const double *src;
+  int64_t ncell;
+  __device__ __forceinline__ void stage_comp(int c) { tp->stage_from(src + c * ncell); }
+  __device__ __forceinline__ double d1(int a) const { return tp->d1(a); }
+  __device__ __forceinline__ double d2(int a) const { return tp->d2(a); }
+  __device__ __forceinline__ double dmix(int a, int b) const { return tp->dmix(a, b); }
+};
+
+template <bool CL>
+__global__ void __launch_bounds__(TX *TY *TZ) k_kappa_tile(Ctx C, const double *__restrict__ us,
+                                                          const double *__restrict__ D,
+                                                          const double *__restrict__ matter, double *kap,
+                                                          TileSet S) {
+  using P = TileProvider<double, TX, TY, TZ, false, CL>;
+  __shared__ double box[P::NBOX];
+  int bx, by, bz, slab;
+  tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz, &slab);
+  const int tid = threadIdx.x;
+  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
+  const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
+  const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;
+  const int yc = y < C.ax[1].n ? y : C.ax[1].n - 1;
+  const int zc = z < C.ax[2].n ? z : C.ax[2].n - 1;
+  const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
+  P tp;
+  tp.setup(C, us, box, box, bx * TX, by * TY, bz * TZ, tx, ty, tz, false);  // no stash, no prefetch
+  KappaTaps<TX, TY, TZ, CL> ps{&tp, us, C.ncell}, pd{&tp, D, C.ncell}, pm{&tp, matter, C.ncell};
+  kappa_eval(C, ps, pd, matter ? &pm : (KappaTaps<TX, TY, TZ, CL> *)nullptr, us, D, matter, idx, kap, inside);
+}
+
 // ------------------------------------------------------------- reductions
 
 constexpr int RED_THREADS = 256;
@@ -357,6 +394,27 @@ int launch_kappa(const Ctx &C, const double *us, const double *matter, double *w
     pts *= B.hi[a] - B.lo[a];
   }
   if (pts <= 0) return BSSN_OK;
+  // the whole grid on tiles when the stage kernels tile it (no MAD blend)
+  const bool whole = pts == C.ncell;
+  const bool mad = C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0;
+  if (whole && !mad && (!C.any_shell || tiles_aligned(C))) {
+    const dim3 block(TX * TY * TZ);
+    if (!C.any_shell) {
+      const TileSet S = tile_set(C, 0, TX, TY, TZ);
+      hipLaunchKernelGGL(k_kappa_tile<false>, dim3(S.count), block, 0, st, C, us, D, matter, kap, S);
+      return check_hip(hipGetLastError(), "kappa tile kernel launch");
+    }
+    const TileSet lean = tile_set(C, 1, TX, TY, TZ), face = tile_set(C, 2, TX, TY, TZ);
+    if (lean.count > 0) {
+      hipLaunchKernelGGL(k_kappa_tile<false>, dim3(lean.count), block, 0, st, C, us, D, matter, kap, lean);
+      if ((rc = check_hip(hipGetLastError(), "kappa tile kernel launch"))) return rc;
+    }
+    if (face.count > 0) {
+      hipLaunchKernelGGL(k_kappa_tile<true>, dim3(face.count), block, 0, st, C, us, D, matter, kap, face);
+      rc = check_hip(hipGetLastError(), "kappa tile kernel launch");
+    }
+    return rc;
+  }
   hipLaunchKernelGGL(k_kappa, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, st, C, us, D, matter, kap, B);
   return check_hip(hipGetLastError(), "kappa kernel launch");
 }

@@ -293,15 +293,15 @@ BSSN_HD inline void derived_point(int64_t n, int64_t i, const double *__restrict
 // 204-218) with M_i and the product-rule d_l M_i of bssn/constraints.py:
 // 22-97 and 100-213.  Mixed second derivatives keep the reference
 // orientation: d2X[l, inner] is the outer derivative l of the inner one.
-BSSN_HD inline void kappa_point(const Ctx &C, const double *__restrict__ us, const double *__restrict__ D,
-                                const double *__restrict__ matter, int64_t i, double *kap) {
-  const int z = (int)(i % C.ax[2].n);
-  const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
-  const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
+// Written once for any provider with stage_comp / d1 / d2 / dmix: ps reads
+// the state, pd the derived fields D, pm the matter sources (may be null).
+// Every provider call is made by every lane (the tile providers stage
+// through LDS with barriers); kap is written only when ``write``.
+template <class P, class PM>
+BSSN_HD __forceinline__ void kappa_eval(const Ctx &C, P &ps, P &pd, PM *pm, const double *__restrict__ us,
+                                        const double *__restrict__ D, const double *__restrict__ matter, int64_t i,
+                                        double *kap, bool write) {
   const int64_t n = C.ncell;
-  PointProvider<double> ps, pd;
-  ps.setup(C, us, x, y, z);
-  pd.setup(C, D, x, y, z);
   double ig[6], B[3][3], BW[3][3];
   for (int c = 0; c < 6; ++c) ig[c] = D[c * n + i];
   for (int r = 0; r < 3; ++r)
@@ -311,7 +311,7 @@ BSSN_HD inline void kappa_point(const Ctx &C, const double *__restrict__ us, con
     }
   const double W = us[CW * n + i], al = us[ALPHA * n + i];
   const double Wf = W > 1.0e-12 ? W : 1.0e-12;
-  auto second = [&](PointProvider<double> &pv, int inner, int outer) -> double {
+  auto second = [&](P &pv, int inner, int outer) -> double {
     return inner == outer ? pv.d2(outer) : pv.dmix(inner, outer);
   };
   double dW[3], dK[3], dA[6][3], dg[6][3];
@@ -363,14 +363,12 @@ BSSN_HD inline void kappa_point(const Ctx &C, const double *__restrict__ us, con
     for (int j = 0; j < 3; ++j) t3 += B[r][j] * dW[j];
     M[r] += -0.5 * t2 - 3.0 * t3 / Wf - 2.0 / 3.0 * dK[r];
   }
-  if (matter) {
+  if (pm) {
     // M_i - 8 pi S_i and d_l M_i - 8 pi d_l S_i (bssn/constraints.py:216-259)
-    PointProvider<double> pm;
-    pm.setup(C, matter, x, y, z);
     for (int r = 0; r < 3; ++r) {
-      pm.stage_comp(1 + r);
+      pm->stage_comp(1 + r);
       M[r] -= 8.0 * M_PI * matter[(1 + r) * n + i];
-      for (int l = 0; l < 3; ++l) dM[r][l] -= 8.0 * M_PI * pm.d1(l);
+      for (int l = 0; l < 3; ++l) dM[r][l] -= 8.0 * M_PI * pm->d1(l);
     }
   }
   // D_j M_i = d_j M_i - Gamma^k_ij M_k
@@ -388,10 +386,24 @@ BSSN_HD inline void kappa_point(const Ctx &C, const double *__restrict__ us, con
       }
       DM[r][j] = dM[r][j] - gm;
     }
+  if (!write) return;
   for (int c = 0; c < 6; ++c) {
     const int a = SYM_I[c], b = SYM_J[c];
     kap[c * n + i] = C.kappa / 2.0 * al * (DM[a][b] + DM[b][a]);
   }
+}
+
+// The kappa term at point i with taps from global memory (any point).
+BSSN_HD inline void kappa_point(const Ctx &C, const double *__restrict__ us, const double *__restrict__ D,
+                                const double *__restrict__ matter, int64_t i, double *kap) {
+  const int z = (int)(i % C.ax[2].n);
+  const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
+  const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
+  PointProvider<double> ps, pd, pm;
+  ps.setup(C, us, x, y, z);
+  pd.setup(C, D, x, y, z);
+  if (matter) pm.setup(C, matter, x, y, z);
+  kappa_eval(C, ps, pd, matter ? &pm : (PointProvider<double> *)nullptr, us, D, matter, i, kap, true);
 }
 
 }  // namespace bssn

@@ -340,6 +340,24 @@ struct TileProvider {
     if (prefetch) fetch(visit_comp(0));
   }
 
+  // Stage one component from an arbitrary array (component base f) into
+  // the first box buffer, synchronously: loads, a barrier (every lane is
+  // done with the previous box), stores, a barrier.  For passes that visit
+  // components of several arrays in their own order (the kappa term).
+  __device__ __forceinline__ void stage_from(const double *f) {
+    const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)f;
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) pref[k] = (T)g[goff[k]];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+      int e = tid + k * NT;
+      if (k < NSLOT - 1 || e < NBOX) lds[e] = pref[k];
+    }
+    __syncthreads();
+    cur = lds;
+  }
+
   // Stage visit v: store the prefetched tile, one barrier, prefetch v + 1.
   // Double-buffered, buffer v & 1 was last read in visit v - 2, which every
   // thread finished before arriving at the barrier of visit v - 1; a single
