@@ -76,6 +76,9 @@ CASES = {
     "zero_shift_harmonic": lambda: (lambda v, p: (v, p._replace(zero_shift=1, gauge=0)))(*shift_case(seed=8)),
     # shapes that tile exactly: closures evaluated inside the LDS tiles
     "sommerfeld_aligned": lambda: sommerfeld_case((16, 16, 32), seed=12),
+    # 24^3: face tiles of every kind (x, y and z slabs with one closure axis,
+    # edges and corners) each take their own kernel
+    "sommerfeld_aligned_all_slabs": lambda: sommerfeld_case((24, 24, 24), seed=15),
     "kappa_periodic": lambda: (lambda v, p: (v, p._replace(kappa=0.3)))(*shift_case(seed=13)),
     "kappa_sommerfeld_aligned": lambda: (lambda v, p: (v, p._replace(kappa=0.2)))(
         *sommerfeld_case((16, 16, 32), seed=14)),
@@ -116,7 +119,7 @@ def test_projection_matches_oracle(bssn):
 
 @pytest.mark.parametrize("case,steps", [("gauge_wave", 6), ("shift_1plog", 4), ("sommerfeld_all", 3),
                                         ("mad_periodic", 2), ("sommerfeld_aligned", 2),
-                                        ("kappa_sommerfeld_aligned", 2)])
+                                        ("kappa_sommerfeld_aligned", 2), ("sommerfeld_aligned_all_slabs", 1)])
 def test_rk4_steps_match_oracle_every_step(bssn, case, steps):
     v, p = CASES[case]()
     print(f"[rk4] case={case} steps={steps}", flush=True)
