@@ -368,6 +368,8 @@ def main():
     ap.add_argument("--no-cartoon", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--no-fmr", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=50,
+                    help="steps of the FP64 / FP32-RHS comparison run (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -400,6 +402,12 @@ def main():
         del ev32
         extra["f32rhs"] = {"value": rate32, "ms_per_step": 1e3 * t32 / args.steps,
                            "speedup_vs_f64": rate32 / rate64, "tile_kernel_hbm_frac": roof32["frac"], **acc}
+        if args.accuracy_steps > 0:
+            # the full 50-step run of the puncture configuration in both precisions
+            e64 = J.Evolver(state.clone(), params, precision="f64").step(args.accuracy_steps)
+            e32 = J.Evolver(state.clone(), params, precision="f32rhs").step(args.accuracy_steps)
+            extra["f32rhs"]["long_run"] = f32_accuracy(e64.state, e32.state, params, args.accuracy_steps)
+            del e64, e32
     del final64
     torch.cuda.synchronize()
     line = {
