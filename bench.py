@@ -313,9 +313,26 @@ def tangent_gradient(J, n=128, steps=16):
     theta = torch.tensor(theta0, requires_grad=True)
     t_e2e, obj = timed(lambda: HamiltonianL2AfterSteps.apply(theta, initial, p, steps))
     obj.backward()
+
+    # central differences of the forward objective at the same size
+    from jax_bssn_amd.constraints import constraints_packed
+
+    def objective(th):
+        u, _ = initial(th)
+        evolve_packed(u, p, steps)
+        h = constraints_packed(u, p)[0]
+        return float(torch.sqrt(torch.mean(h * h)))
+    fd = []
+    for k, h in ((0, 1e-4), (1, 1e-4)):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd.append((objective(tp) - objective(tm)) / (2 * h))
     return {"method": "forward-mode tangent (dual numbers), both parameters in one run", "grid": [n, n, n],
             "steps": steps, "parameters": ["M", "eps"], "hamiltonian_l2": val, "gradient": list(grad),
-            "autograd_gradient": theta.grad.tolist(), "forward_seconds": t_fwd,
+            "autograd_gradient": theta.grad.tolist(), "central_difference_gradient": fd,
+            "rel_diff_vs_central_difference": [abs(g - f) / abs(f) for g, f in zip(grad, fd)],
+            "forward_seconds": t_fwd,
             "value_and_gradient_seconds": t_grad, "gradient_forward_ratio": t_grad / t_fwd,
             "autograd_call_seconds": t_e2e}
 
