@@ -453,6 +453,7 @@ def main():
     }
     if not args.no_batch:
         extra["batched_gauge_waves"] = batched_gauge_waves(J, args.steps)
+        extra["batched_gauge_waves_f32rhs"] = batched_gauge_waves(J, args.steps, precision="f32rhs")
     if not args.no_cartoon:
         extra["axisymmetric_puncture"] = axisymmetric_puncture(J, args.steps)
     if not args.no_fmr:

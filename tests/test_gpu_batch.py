@@ -105,3 +105,23 @@ def test_unsupported_configurations_raise(J):
         BatchEvolver(states, [params[0]._replace(xl_bc=1, xr_bc=1)], tile_variant=0, grid_blocks=64)
     with pytest.raises(NotImplementedError, match="kappa"):
         BatchEvolver(states, [params[0]._replace(kappa=0.1)], tile_variant=0, grid_blocks=64)
+
+
+def test_f32rhs_batch_matches_single_grid_f32rhs(J):
+    """FP32-RHS batched steps equal the FP32-RHS single-grid steps of each
+    instance (the same per-point code; 1e-12 allows for contraction
+    differences between the two kernels), and stay within the FP32-RHS error
+    of the FP64 evolution."""
+    from jax_bssn_amd.batch import BatchEvolver
+    states, params, _ = gauge_batch(J, (16, 24, 32), seed=11)
+    b = BatchEvolver([s.clone() for s in states], params, precision="f32rhs", tile_variant=0, grid_blocks=256)
+    b.step(2)
+    for s, p, bs in zip(states, params, b.states):
+        e32 = J.Evolver(s.clone(), p, precision="f32rhs").step(2)
+        e64 = J.Evolver(s.clone(), p).step(2)
+        scale = float(e64.state.abs().max())
+        d32 = float((bs - e32.state).abs().max()) / scale
+        d64 = float((bs - e64.state).abs().max()) / scale
+        print("f32rhs batch vs single f32rhs", f"{d32:.1e}", "vs f64", f"{d64:.1e}", flush=True)
+        assert d32 <= 1e-12
+        assert d64 <= 1e-5
