@@ -235,7 +235,7 @@ def fmr_linear_wave(J, steps, n=64, levels=3, amplitude=1.0e-8):
             "finest_gamma_xy_max": float(h.abs().max())}
 
 
-def axisymmetric_puncture(J, steps, nr=512, nz=1024, separation=1.5, precision="f64"):
+def axisymmetric_puncture(J, steps, nr=512, nz=1024, separation=1.5, precision="f64", keep_state=False):
     """Head-on pair of zero-momentum punctures (m = 0.5 at z = +-s/2) on the
     compact (rho, z) plane nr x nz (+ 4 parity ghosts), dx = 1/32, moving-
     puncture gauge (1+log, Gamma driver eta = 2, kappa = 0.002, KO nu = 0.02),
@@ -264,9 +264,11 @@ def axisymmetric_puncture(J, steps, nr=512, nz=1024, separation=1.5, precision="
     e1.record()
     torch.cuda.synchronize()
     t = e0.elapsed_time(e1) / 1e3
-    return {"value": nr * nz * steps / t, "unit": "cell-updates/s (compact points)", "compact": [nr, nz],
-            "ms_per_step": 1e3 * t / steps, "precision": precision,
-            "finite": bool(torch.isfinite(ev.u).all())}
+    out = {"value": nr * nz * steps / t, "unit": "cell-updates/s (compact points)", "compact": [nr, nz],
+           "ms_per_step": 1e3 * t / steps, "precision": precision, "finite": bool(torch.isfinite(ev.u).all())}
+    if keep_state:
+        out["state"] = ev.u
+    return out
 
 
 def tangent_gradient(J, n=128, steps=16):
@@ -455,7 +457,14 @@ def main():
         extra["batched_gauge_waves"] = batched_gauge_waves(J, args.steps)
         extra["batched_gauge_waves_f32rhs"] = batched_gauge_waves(J, args.steps, precision="f32rhs")
     if not args.no_cartoon:
-        extra["axisymmetric_puncture"] = axisymmetric_puncture(J, args.steps)
+        a64 = axisymmetric_puncture(J, args.steps, keep_state=True)
+        a32 = axisymmetric_puncture(J, args.steps, precision="f32rhs", keep_state=True)
+        s64, s32 = a64.pop("state"), a32.pop("state")
+        a32["max_rel_diff_vs_f64"] = float((s32 - s64).abs().max() / s64.abs().max())
+        a32["speedup_vs_f64"] = a32["value"] / a64["value"]
+        del s64, s32
+        extra["axisymmetric_puncture"] = a64
+        extra["axisymmetric_puncture_f32rhs"] = a32
     if not args.no_fmr:
         extra["fmr_linear_wave"] = fmr_linear_wave(J, args.steps)
     if not args.no_gradient:

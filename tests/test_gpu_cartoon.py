@@ -93,6 +93,19 @@ def test_compact_rhs_matches_oracle(C, z_somm, kappa):
     assert max(errs.values()) <= 1e-10
 
 
+def test_compact_rhs_f32rhs_within_fp32_error(C):
+    """The FP32-RHS variant of the centre-plane RHS (derivatives and algebra
+    in FP32 on values minus the point's own FP64 value) against the FP64
+    oracle: within 1e-4 of each field's scale (FP32 has 6e-8 unit roundoff;
+    the RHS combines O(10^2) terms of the derivative scale)."""
+    v, p = case(z_somm=True)
+    got = C.compute_axisymmetric_rhs(_jv(v), _jp(p), precision="f32rhs")
+    want = OC.compute_axisymmetric_rhs(v, p)
+    errs = rel_errors(got, want)
+    print("axi rhs f32rhs", " ".join(f"{k}={e:.1e}" for k, e in errs.items()), flush=True)
+    assert max(errs.values()) <= 1e-4
+
+
 @pytest.mark.parametrize("z_somm", [False, True])
 def test_axisymmetric_steps_match_oracle(C, z_somm):
     v, p = case(z_somm=z_somm)
