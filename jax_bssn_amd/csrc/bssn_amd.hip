@@ -249,9 +249,9 @@ __global__ void __launch_bounds__(256) k_kappa(Ctx C, const double *__restrict__
 // The kappa term on tiles: the components of the state, the derived fields
 // and the matter sources are staged one at a time in one LDS box (the tile
 // stencils, closure rows on face tiles), instead of global-memory taps.
-template <int TX_, int TY_, int TZ_, bool CL>
+template <class P>
 struct KappaTaps {
-  TileProvider<double, TX_, TY_, TZ_, false, CL> *tp;
+  P *tp;
   const double *src;
   int64_t ncell;
   __device__ __forceinline__ void stage_comp(int c) { tp->stage_from(src + c * ncell); }
@@ -279,8 +279,38 @@ __global__ void __launch_bounds__(TX *TY *TZ) k_kappa_tile(Ctx C, const double *
   const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
   P tp;
   tp.setup(C, us, box, box, bx * TX, by * TY, bz * TZ, tx, ty, tz, false);  // no stash, no prefetch
-  KappaTaps<TX, TY, TZ, CL> ps{&tp, us, C.ncell}, pd{&tp, D, C.ncell}, pm{&tp, matter, C.ncell};
-  kappa_eval(C, ps, pd, matter ? &pm : (KappaTaps<TX, TY, TZ, CL> *)nullptr, us, D, matter, idx, kap, inside);
+  KappaTaps<P> ps{&tp, us, C.ncell}, pd{&tp, D, C.ncell}, pm{&tp, matter, C.ncell};
+  kappa_eval(C, ps, pd, matter ? &pm : (KappaTaps<P> *)nullptr, us, D, matter, idx, kap, inside);
+}
+
+// The kappa term on a box one plane thick along y (the Cartoon centre
+// plane): 8 x 1 x 32 tiles over [lo, hi), closure rows along x and z.
+constexpr int RTX = 8, RTY = 1, RTZ = 32;
+
+struct RegionTiles {
+  int lo[3], hi[3];
+  int nz;  // tiles along z
+};
+
+__global__ void __launch_bounds__(RTX *RTY *RTZ) k_kappa_region(Ctx C, const double *__restrict__ us,
+                                                               const double *__restrict__ D,
+                                                               const double *__restrict__ matter, double *kap,
+                                                               RegionTiles R) {
+  using P = TileProvider<double, RTX, RTY, RTZ, false, true, 5>;
+  __shared__ double box[P::NBOX];
+  const int tile = (int)blockIdx.x;
+  const int x0 = R.lo[0] + (tile / R.nz) * RTX, y0 = R.lo[1], z0 = R.lo[2] + (tile % R.nz) * RTZ;
+  const int tid = threadIdx.x;
+  const int tz = tid % RTZ, tx = tid / RTZ;
+  const int x = x0 + tx, z = z0 + tz;
+  const bool inside = x < R.hi[0] && z < R.hi[2];
+  const int xc = x < R.hi[0] ? x : R.hi[0] - 1;
+  const int zc = z < R.hi[2] ? z : R.hi[2] - 1;
+  const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)y0 * C.ax[1].stride + zc;
+  P tp;
+  tp.setup(C, us, box, box, x0, y0, z0, tx, 0, tz, false);  // no stash, no prefetch
+  KappaTaps<P> ps{&tp, us, C.ncell}, pd{&tp, D, C.ncell}, pm{&tp, matter, C.ncell};
+  kappa_eval(C, ps, pd, matter ? &pm : (KappaTaps<P> *)nullptr, us, D, matter, idx, kap, inside);
 }
 
 // ------------------------------------------------------------- reductions
@@ -414,6 +444,16 @@ int launch_kappa(const Ctx &C, const double *us, const double *matter, double *w
       rc = check_hip(hipGetLastError(), "kappa tile kernel launch");
     }
     return rc;
+  }
+  // one plane along y, no closures along y: region tiles
+  if (!mad && B.hi[1] - B.lo[1] == 1 && !C.ax[1].lbc && !C.ax[1].rbc && !C.ax[1].reduced) {
+    RegionTiles R;
+    for (int a = 0; a < 3; ++a) R.lo[a] = B.lo[a], R.hi[a] = B.hi[a];
+    const int nx = (B.hi[0] - B.lo[0] + RTX - 1) / RTX;
+    R.nz = (B.hi[2] - B.lo[2] + RTZ - 1) / RTZ;
+    hipLaunchKernelGGL(k_kappa_region, dim3((unsigned)(nx * R.nz)), dim3(RTX * RTY * RTZ), 0, st, C, us, D, matter,
+                       kap, R);
+    return check_hip(hipGetLastError(), "kappa region kernel launch");
   }
   hipLaunchKernelGGL(k_kappa, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, st, C, us, D, matter, kap, B);
   return check_hip(hipGetLastError(), "kappa kernel launch");
