@@ -284,14 +284,7 @@ __global__ void __launch_bounds__(TX *TY *TZ) k_kappa_tile(Ctx C, const double *
 }
 
 // The kappa term on a box one plane thick along y (the Cartoon centre
-// plane): 8 x 1 x 32 tiles over [lo, hi), closure rows along x and z.
-constexpr int RTX = 8, RTY = 1, RTZ = 32;
-
-struct RegionTiles {
-  int lo[3], hi[3];
-  int nz;  // tiles along z
-};
-
+// plane), on the region tiles of stage_kernels.h.
 __global__ void __launch_bounds__(RTX *RTY *RTZ) k_kappa_region(Ctx C, const double *__restrict__ us,
                                                                const double *__restrict__ D,
                                                                const double *__restrict__ matter, double *kap,
@@ -446,13 +439,11 @@ int launch_kappa(const Ctx &C, const double *us, const double *matter, double *w
     return rc;
   }
   // one plane along y, no closures along y: region tiles
-  if (!mad && B.hi[1] - B.lo[1] == 1 && !C.ax[1].lbc && !C.ax[1].rbc && !C.ax[1].reduced) {
-    RegionTiles R;
-    for (int a = 0; a < 3; ++a) R.lo[a] = B.lo[a], R.hi[a] = B.hi[a];
-    const int nx = (B.hi[0] - B.lo[0] + RTX - 1) / RTX;
-    R.nz = (B.hi[2] - B.lo[2] + RTZ - 1) / RTZ;
-    hipLaunchKernelGGL(k_kappa_region, dim3((unsigned)(nx * R.nz)), dim3(RTX * RTY * RTZ), 0, st, C, us, D, matter,
-                       kap, R);
+  if (region_tiled(C, B.lo, B.hi)) {
+    int count = 0;
+    const RegionTiles R = region_tiles(B.lo, B.hi, &count);
+    hipLaunchKernelGGL(k_kappa_region, dim3((unsigned)count), dim3(RTX * RTY * RTZ), 0, st, C, us, D, matter, kap,
+                       R);
     return check_hip(hipGetLastError(), "kappa region kernel launch");
   }
   hipLaunchKernelGGL(k_kappa, dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, st, C, us, D, matter, kap, B);
@@ -486,6 +477,11 @@ int launch_region(const Ctx &C, int precision, const double *us, double *rhs, co
     const int rc = launch_kappa(C, us, nullptr, work, st, lo, hi);
     if (rc) return rc;
     O.kap = work + (int64_t)NDERIVED * C.ncell;
+  }
+  if (region_tiled(C, lo, hi) && (int64_t)(hi[0] - lo[0]) * (hi[2] - lo[2]) > 0) {
+    if (precision == BSSN_PREC_F64) return launch_region_f64(C, us, O, lo, hi, st);
+    if (precision == BSSN_PREC_F32RHS) return launch_region_f32(C, us, O, lo, hi, st);
+    return fail(BSSN_ERR_INVALID, "unknown precision %d", precision);
   }
   Slabs S;
   std::memset(&S, 0, sizeof S);

@@ -11,4 +11,9 @@ int launch_slabs_f64(const Ctx &C, const double *us, const StageOut &O, const Sl
   return launch_slabs_t<double>(C, us, O, S, st);
 }
 
+int launch_region_f64(const Ctx &C, const double *us, const StageOut &O, const int *lo, const int *hi,
+                      hipStream_t st) {
+  return launch_region_t<double>(C, us, O, lo, hi, st);
+}
+
 }  // namespace bssn

@@ -78,6 +78,77 @@ __global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
   bssn_rhs_point<T>(pv, C, u, out);
 }
 
+// ------------------------------------------------------- region tiles
+
+// A box one plane thick along y (the Cartoon centre plane) as 8 x 1 x 32
+// tiles over [lo, hi), tile t at lo + ((t / nz) * 8, 0, (t % nz) * 32),
+// with closure rows along x and z (the box may end on those faces).
+constexpr int RTX = 8, RTY = 1, RTZ = 32;
+
+struct RegionTiles {
+  int lo[3], hi[3];
+  int nz;  // tiles along z
+};
+
+inline RegionTiles region_tiles(const int *lo, const int *hi, int *count) {
+  RegionTiles R;
+  for (int a = 0; a < 3; ++a) R.lo[a] = lo[a], R.hi[a] = hi[a];
+  const int nx = (hi[0] - lo[0] + RTX - 1) / RTX;
+  R.nz = (hi[2] - lo[2] + RTZ - 1) / RTZ;
+  *count = nx * R.nz;
+  return R;
+}
+
+// Region tiles apply: no MAD blend, one plane along y, no closures along y,
+// and along x and z the tile grid anchored at lo meets every closure face
+// on a tile boundary (then a face tile spans the face's last 8 or 32
+// points, and its closure rows, which reach 7 points inward, stay inside
+// the box of the tile plus its 3-point halo).
+inline bool region_tiled(const Ctx &C, const int *lo, const int *hi) {
+  const bool mad = C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0;
+  if (mad || hi[1] - lo[1] != 1 || C.ax[1].lbc || C.ax[1].rbc || C.ax[1].reduced) return false;
+  const int t[3] = {RTX, RTY, RTZ};
+  for (int a = 0; a < 3; a += 2) {
+    const Axis &A = C.ax[a];
+    if (A.lbc && (lo[a] - A.start) % t[a] != 0) return false;
+    if (A.rbc && (A.end - lo[a]) % t[a] != 0) return false;
+  }
+  return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RTX *RTY *RTZ, BSSN_MIN_WAVES)
+    k_stage_region(Ctx C, const double *__restrict__ us, StageOut O, RegionTiles R) {
+  using P = TileProvider<T, RTX, RTY, RTZ, false, true, 5>;
+  __shared__ T smem[P::LDS_ELEMS];
+  __shared__ double stash[STASH_SLOTS * RTX * RTY * RTZ];
+  const int tile = (int)blockIdx.x;
+  const int x0 = R.lo[0] + (tile / R.nz) * RTX, y0 = R.lo[1], z0 = R.lo[2] + (tile % R.nz) * RTZ;
+  const int tid = threadIdx.x;
+  const int tz = tid % RTZ, tx = tid / RTZ;
+  const int x = x0 + tx, z = z0 + tz;
+  const bool inside = x < R.hi[0] && z < R.hi[2];
+  const int xc = x < R.hi[0] ? x : R.hi[0] - 1;
+  const int zc = z < R.hi[2] ? z : R.hi[2] - 1;
+  const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)y0 * C.ax[1].stride + zc;
+  P pv;
+  pv.setup(C, us, smem, stash, x0, y0, z0, tx, 0, tz);
+  double u[NCOMP];
+  load_point(C, us, idx, u);
+  Emitter<LdsStash<RTX * RTY * RTZ>> out(C, O, idx, inside, pv.face(), pv.st);
+  bssn_rhs_point<T>(pv, C, u, out);
+}
+
+template <typename T>
+inline int launch_region_t(const Ctx &C, const double *us, const StageOut &O, const int *lo, const int *hi,
+                           hipStream_t st) {
+  int count = 0;
+  const RegionTiles R = region_tiles(lo, hi, &count);
+  if (count <= 0) return BSSN_OK;
+  hipLaunchKernelGGL((k_stage_region<T>), dim3(count), dim3(RTX * RTY * RTZ), 0, st, C, us, O, R);
+  return check_hip(hipGetLastError(), "region tile kernel launch");
+}
+
 // ----------------------------------------------------------- shell kernel
 
 __device__ __forceinline__ bool in_slab(const Slabs &S, int s, int x, int y, int z) {
@@ -195,6 +266,9 @@ int launch_stage_f32(const Ctx &C, const double *us, const StageOut &O, hipStrea
 // the closure-capable point kernel over a list of slabs
 int launch_slabs_f64(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st);
 int launch_slabs_f32(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st);
+// the region tile kernel (region_tiled() boxes)
+int launch_region_f64(const Ctx &C, const double *us, const StageOut &O, const int *lo, const int *hi, hipStream_t st);
+int launch_region_f32(const Ctx &C, const double *us, const StageOut &O, const int *lo, const int *hi, hipStream_t st);
 
 template <typename T>
 inline int launch_slabs_t(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st) {

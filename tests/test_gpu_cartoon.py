@@ -93,6 +93,24 @@ def test_compact_rhs_matches_oracle(C, z_somm, kappa):
     assert max(errs.values()) <= 1e-10
 
 
+@pytest.mark.parametrize("kappa", [0.0, 0.02])
+def test_compact_rhs_on_region_tiles_matches_oracle(C, kappa):
+    """full_nx = 32: 16 radial points after the 4 axis ghosts and 32 z
+    points, so the centre plane splits into 8 x 1 x 32 tiles that meet the
+    Sommerfeld faces on tile boundaries (the region tile kernels of the RHS
+    and of the kappa term; the other cases take the point kernels)."""
+    v, p = case(full_nx=32, z_somm=True, kappa=kappa)
+    got = C.compute_axisymmetric_rhs(_jv(v), _jp(p))
+    want = OC.compute_axisymmetric_rhs(v, p)
+    errs = rel_errors(got, want)
+    print("axi rhs region tiles", kappa, " ".join(f"{k}={e:.1e}" for k, e in errs.items()), flush=True)
+    assert max(errs.values()) <= 1e-10
+    got32 = C.compute_axisymmetric_rhs(_jv(v), _jp(p), precision="f32rhs")
+    errs32 = rel_errors(got32, want)
+    print("axi rhs region tiles f32rhs", kappa, " ".join(f"{k}={e:.1e}" for k, e in errs32.items()), flush=True)
+    assert max(errs32.values()) <= 1e-4
+
+
 def test_compact_rhs_f32rhs_within_fp32_error(C):
     """The FP32-RHS variant of the centre-plane RHS (derivatives and algebra
     in FP32 on values minus the point's own FP64 value) against the FP64
