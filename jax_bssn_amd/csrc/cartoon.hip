@@ -75,7 +75,7 @@ __device__ __forceinline__ double axi_source(const AxiGeom &G, const double *__r
   return a + (f[(int64_t)(G.nx - 1) * G.nz + k] - a) * ratio_b[m - G.nx];
 }
 
-__global__ void __launch_bounds__(256) k_axi_reconstruct(AxiGeom G, const double *__restrict__ u,
+__global__ void __launch_bounds__(256, 2) k_axi_reconstruct(AxiGeom G, const double *__restrict__ u,
                                                          double *__restrict__ sup) {
   const int64_t nsup = (int64_t)G.nx * AX_SUPPORT * G.nz;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,8 +113,10 @@ __global__ void __launch_bounds__(256) k_axi_reconstruct(AxiGeom G, const double
     w[a] = p;
   }
   const bool valid = q >= 0.0 && q <= (double)(size - 1);
+  // every loop over components is unrolled so that prof / out stay in
+  // registers (a runtime index would place them in scratch memory)
   double prof[NCOMP];
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < NCOMP; ++c) {
     double v = 0.0;
 #pragma unroll
@@ -128,12 +130,14 @@ __global__ void __launch_bounds__(256) k_axi_reconstruct(AxiGeom G, const double
 #pragma unroll
   for (int c = 0; c < NCOMP; ++c) out[c] = prof[c];
   const int vecs[3] = {GAM, BETA, -1};
+#pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int b = vecs[t];
 #pragma unroll
     for (int r = 0; r < 3; ++r) out[b + r] = L[r][0] * prof[b] + L[r][1] * prof[b + 1] + L[r][2] * prof[b + 2];
   }
   const int tens[2] = {GXX, AXX};
+#pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int b = tens[t];
     double T[3][3];
@@ -153,7 +157,7 @@ __global__ void __launch_bounds__(256) k_axi_reconstruct(AxiGeom G, const double
         out[b + sym(r, s)] = acc;
       }
   }
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < NCOMP; ++c) sup[c * nsup + id] = out[c];
 }
 
