@@ -37,20 +37,27 @@ struct ProlongGeom {
   int active[3];   // child active extents
 };
 
-// Six Lagrange nodes and weights of child index i along one axis.
-__device__ __forceinline__ void axis_stencil(const ProlongGeom &G, int a, int i, int node[6], double w[6]) {
+// Six-node Lagrange weights at parent coordinate t of child index i along
+// axis a; the nodes are first, first + 1, .., first + 5 (unwrapped).
+__device__ __forceinline__ void lagrange6(const ProlongGeom &G, int a, int i, int &first, double w[6]) {
   const double t = (double)(G.poff + G.lo[a]) + (double)(i - G.g) * 0.5;
-  const int base = (int)floor(t);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) node[k] = base - 2 + k;
+  first = (int)floor(t) - 2;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     double p = 1.0;
 #pragma unroll
     for (int m = 0; m < 6; ++m)
-      if (m != k) p *= (t - (double)node[m]) / (double)(node[k] - node[m]);
+      if (m != k) p *= (t - (double)(first + m)) / (double)(k - m);
     w[k] = p;
   }
+}
+
+// Six Lagrange nodes and weights of child index i along one axis.
+__device__ __forceinline__ void axis_stencil(const ProlongGeom &G, int a, int i, int node[6], double w[6]) {
+  int first;
+  lagrange6(G, a, i, first, w);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) node[k] = first + k;
   const int n = G.parent.n[a];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -72,9 +79,11 @@ __host__ __device__ __forceinline__ int64_t shell_points(const ProlongGeom &G) {
          (int64_t)G.active[0] * G.active[1] * g2;
 }
 
+// Shell points from r0 on (r0 = 0: the whole shell; r0 = the x- and y-slab
+// count: the z slabs only).
 __global__ void __launch_bounds__(256) k_fmr_prolong(ProlongGeom G, const double *__restrict__ parent,
-                                                     double *__restrict__ child) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                     double *__restrict__ child, int64_t r0) {
+  int64_t r = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= shell_points(G)) return;
   const int g = G.g, g2 = 2 * G.g, NY = G.child.n[1], NZ = G.child.n[2];
   const int64_t nxs = (int64_t)g2 * NY * NZ, nys = (int64_t)G.active[0] * g2 * NZ;
@@ -118,6 +127,74 @@ __global__ void __launch_bounds__(256) k_fmr_prolong(ProlongGeom G, const double
       v += wz[cz] * vy;
     }
     child[c * G.child.ncell + id] = v;
+  }
+}
+
+// The rows of the x and y slabs are whole z rows of ghosts, with x and y
+// weights common to the row.  One workgroup per row interpolates the parent
+// in x and y once per parent z node of the row (into LDS), then every child
+// point of the row takes its six z nodes from there: about 36 / 2 + 6
+// parent reads per point and component instead of 216.  The sums run in
+// the order of k_fmr_prolong (x innermost, then y, then z).
+constexpr int ROW_THREADS = 64;
+constexpr int ROW_MAXV = 1024;  // parent z nodes of one row (NZ / 2 + 7 at most)
+
+__global__ void __launch_bounds__(ROW_THREADS) k_fmr_prolong_rows(ProlongGeom G, const double *__restrict__ parent,
+                                                                  double *__restrict__ child) {
+  __shared__ double V[ROW_MAXV];
+  const int q = (int)blockIdx.x;
+  const int g = G.g, NY = G.child.n[1], NZ = G.child.n[2];
+  const int rows_x = 2 * g * NY;
+  int i, j;
+  if (q < rows_x) {
+    const int t = q / NY;
+    j = q % NY;
+    i = t < g ? t : t + G.active[0];
+  } else {
+    const int qq = q - rows_x, t = qq % (2 * g);
+    i = g + qq / (2 * g);
+    j = t < g ? t : t + G.active[1];
+  }
+  int nx[6], ny[6];
+  double wx[6], wy[6];
+  axis_stencil(G, 0, i, nx, wx);
+  axis_stencil(G, 1, j, ny, wy);
+  int zb0, zb1;
+  double wz[6];
+  lagrange6(G, 2, 0, zb0, wz);
+  lagrange6(G, 2, NZ - 1, zb1, wz);
+  const int nv = zb1 - zb0 + 6;  // parent z nodes zb0 .. zb1 + 5
+  const int64_t sx = (int64_t)G.parent.n[1] * G.parent.n[2], sy = G.parent.n[2];
+  const int pn = G.parent.n[2];
+  const int64_t crow = ((int64_t)i * NY + j) * NZ;
+  for (int c = 0; c < NCOMP; ++c) {
+    const double *P = parent + c * G.parent.ncell;
+    __syncthreads();  // every lane is done with the previous component's nodes
+    for (int v = threadIdx.x; v < nv; v += ROW_THREADS) {
+      int zp = zb0 + v;
+      if (G.periodic[2]) {
+        zp %= pn;
+        if (zp < 0) zp += pn;
+      }
+      double vy = 0.0;
+      for (int by = 0; by < 6; ++by) {
+        const double *row = P + ny[by] * sy + zp;
+        double vx = 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 6; ++ax) vx += wx[ax] * row[nx[ax] * sx];
+        vy += wy[by] * vx;
+      }
+      V[v] = vy;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NZ; k += ROW_THREADS) {
+      int first;
+      lagrange6(G, 2, k, first, wz);
+      double v = 0.0;
+#pragma unroll
+      for (int cz = 0; cz < 6; ++cz) v += wz[cz] * V[first - zb0 + cz];
+      child[c * G.child.ncell + crow + k] = v;
+    }
   }
 }
 
@@ -201,9 +278,21 @@ int bssn_fmr_prolong(const bssn_grid *parent_grid, int32_t parent_ghost, const i
         return fail(BSSN_ERR_INVALID, "child prolongation stencil exceeds parent allocation on axis %d", a);
     }
   }
-  const int64_t blocks = (shell_points(G) + 255) / 256;
+  const int64_t total = shell_points(G);
+  if (total == 0) return BSSN_OK;
+  // x and y slabs row by row when a row's parent z nodes fit the LDS
+  // buffer, then the z slabs point by point
+  const int rows = 2 * G.g * G.child.n[1] + G.active[0] * 2 * G.g;
+  int64_t r0 = 0;
+  if (rows > 0 && G.child.n[2] / 2 + 7 <= ROW_MAXV) {
+    hipLaunchKernelGGL(k_fmr_prolong_rows, dim3((unsigned)rows), dim3(ROW_THREADS), 0, (hipStream_t)stream, G, parent,
+                       child);
+    if ((rc = check_hip(hipGetLastError(), "prolongation row kernel launch"))) return rc;
+    r0 = (int64_t)rows * G.child.n[2];
+  }
+  const int64_t blocks = (total - r0 + 255) / 256;
   if (blocks == 0) return BSSN_OK;
-  hipLaunchKernelGGL(k_fmr_prolong, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, G, parent, child);
+  hipLaunchKernelGGL(k_fmr_prolong, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, G, parent, child, r0);
   return check_hip(hipGetLastError(), "prolongation kernel launch");
 }
 
