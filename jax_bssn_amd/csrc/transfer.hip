@@ -137,6 +137,12 @@ __global__ void __launch_bounds__(256) k_fmr_prolong(ProlongGeom G, const double
 // parent reads per point and component instead of 216.  The sums run in
 // the order of k_fmr_prolong (x innermost, then y, then z).
 constexpr int ROW_THREADS = 64;
+// A workgroup of the row and column kernels takes COMPS_PER_BLOCK of the
+// components (blockIdx.y picks which): one row or column group is a single
+// wave, and the split gives the launch enough waves to hide the latency of
+// the parent reads.
+constexpr int COMPS_PER_BLOCK = 3;
+static_assert(NCOMP % COMPS_PER_BLOCK == 0, "components split evenly");
 constexpr int ROW_MAXV = 1024;  // parent z nodes of one row (NZ / 2 + 7 at most)
 
 __global__ void __launch_bounds__(ROW_THREADS) k_fmr_prolong_rows(ProlongGeom G, const double *__restrict__ parent,
@@ -167,7 +173,7 @@ __global__ void __launch_bounds__(ROW_THREADS) k_fmr_prolong_rows(ProlongGeom G,
   const int64_t sx = (int64_t)G.parent.n[1] * G.parent.n[2], sy = G.parent.n[2];
   const int pn = G.parent.n[2];
   const int64_t crow = ((int64_t)i * NY + j) * NZ;
-  for (int c = 0; c < NCOMP; ++c) {
+  for (int c = (int)blockIdx.y * COMPS_PER_BLOCK; c < ((int)blockIdx.y + 1) * COMPS_PER_BLOCK; ++c) {
     const double *P = parent + c * G.parent.ncell;
     __syncthreads();  // every lane is done with the previous component's nodes
     for (int v = threadIdx.x; v < nv; v += ROW_THREADS) {
@@ -194,6 +200,68 @@ __global__ void __launch_bounds__(ROW_THREADS) k_fmr_prolong_rows(ProlongGeom G,
 #pragma unroll
       for (int cz = 0; cz < 6; ++cz) v += wz[cz] * V[first - zb0 + cz];
       child[c * G.child.ncell + crow + k] = v;
+    }
+  }
+}
+
+// The z slabs are short z columns: 2 g ghosts per (i, j) of the active x-y
+// range, g at each end.  Sixteen lanes take one column: lanes [0, nlo)
+// interpolate the parent in x and y at the parent z nodes of the low ghosts
+// (first node zlo), lanes [nlo, nlo + nhi) at those of the high ghosts
+// (first node zhi), into LDS; then lanes [0, 2 g) combine six of them in z
+// per ghost.  Four columns per 64-lane workgroup.  Same sum order as
+// k_fmr_prolong.
+constexpr int COL_LANES = 16;
+constexpr int COL_PER_BLOCK = 64 / COL_LANES;
+
+__global__ void __launch_bounds__(64) k_fmr_prolong_zcols(ProlongGeom G, const double *__restrict__ parent,
+                                                          double *__restrict__ child, int zlo, int nlo, int zhi,
+                                                          int nhi) {
+  __shared__ double V[COL_PER_BLOCK][COL_LANES];
+  const int lane = (int)threadIdx.x % COL_LANES, slot = (int)threadIdx.x / COL_LANES;
+  const int col = (int)blockIdx.x * COL_PER_BLOCK + slot;
+  const bool valid = col < G.active[0] * G.active[1];
+  const int g = G.g, NY = G.child.n[1], NZ = G.child.n[2];
+  const int i = g + (valid ? col / G.active[1] : 0), j = g + (valid ? col % G.active[1] : 0);
+  int nx[6], ny[6];
+  double wx[6], wy[6], wz[6];
+  axis_stencil(G, 0, i, nx, wx);
+  axis_stencil(G, 1, j, ny, wy);
+  // node of this lane (phase 1) and ghost of this lane (phase 2)
+  const bool node_lane = valid && lane < nlo + nhi;
+  int zp = lane < nlo ? zlo + lane : zhi + (lane - nlo);
+  const int pn = G.parent.n[2];
+  if (G.periodic[2]) {
+    zp %= pn;
+    if (zp < 0) zp += pn;
+  }
+  const bool ghost_lane = valid && lane < 2 * g;
+  const int k = lane < g ? lane : lane + G.active[2];
+  int first = 0;
+  if (ghost_lane) lagrange6(G, 2, k, first, wz);
+  const int vbase = lane < g ? first - zlo : nlo + first - zhi;
+  const int64_t sx = (int64_t)G.parent.n[1] * G.parent.n[2], sy = G.parent.n[2];
+  const int64_t id = ((int64_t)i * NY + j) * NZ + k;
+  for (int c = (int)blockIdx.y * COMPS_PER_BLOCK; c < ((int)blockIdx.y + 1) * COMPS_PER_BLOCK; ++c) {
+    __syncthreads();  // every lane is done with the previous component's nodes
+    if (node_lane) {
+      const double *P = parent + c * G.parent.ncell;
+      double vy = 0.0;
+      for (int by = 0; by < 6; ++by) {
+        const double *row = P + ny[by] * sy + zp;
+        double vx = 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 6; ++ax) vx += wx[ax] * row[nx[ax] * sx];
+        vy += wy[by] * vx;
+      }
+      V[slot][lane] = vy;
+    }
+    __syncthreads();
+    if (ghost_lane) {
+      double v = 0.0;
+#pragma unroll
+      for (int cz = 0; cz < 6; ++cz) v += wz[cz] * V[slot][vbase + cz];
+      child[c * G.child.ncell + id] = v;
     }
   }
 }
@@ -281,14 +349,27 @@ int bssn_fmr_prolong(const bssn_grid *parent_grid, int32_t parent_ghost, const i
   const int64_t total = shell_points(G);
   if (total == 0) return BSSN_OK;
   // x and y slabs row by row when a row's parent z nodes fit the LDS
-  // buffer, then the z slabs point by point
+  // buffer, then the z slabs column by column when a column's parent z
+  // nodes fit its sixteen lanes; what is left point by point
   const int rows = 2 * G.g * G.child.n[1] + G.active[0] * 2 * G.g;
   int64_t r0 = 0;
   if (rows > 0 && G.child.n[2] / 2 + 7 <= ROW_MAXV) {
-    hipLaunchKernelGGL(k_fmr_prolong_rows, dim3((unsigned)rows), dim3(ROW_THREADS), 0, (hipStream_t)stream, G, parent,
-                       child);
+    const unsigned groups = NCOMP / COMPS_PER_BLOCK;
+    hipLaunchKernelGGL(k_fmr_prolong_rows, dim3((unsigned)rows, groups), dim3(ROW_THREADS), 0, (hipStream_t)stream, G,
+                       parent, child);
     if ((rc = check_hip(hipGetLastError(), "prolongation row kernel launch"))) return rc;
     r0 = (int64_t)rows * G.child.n[2];
+    // first parent z node of child z index k (as lagrange6 forms it)
+    auto zfirst = [&](int k) { return (int)std::floor((double)(G.poff + G.lo[2]) + (double)(k - G.g) * 0.5) - 2; };
+    const int zlo = zfirst(0), nlo = zfirst(G.g - 1) - zlo + 6;
+    const int zhi = zfirst(G.g + G.active[2]), nhi = zfirst(G.child.n[2] - 1) - zhi + 6;
+    const int cols = G.active[0] * G.active[1];
+    if (r0 + (int64_t)cols * 2 * G.g == total && 2 * G.g <= COL_LANES && nlo + nhi <= COL_LANES) {
+      const unsigned blocks = (unsigned)((cols + COL_PER_BLOCK - 1) / COL_PER_BLOCK);
+      hipLaunchKernelGGL(k_fmr_prolong_zcols, dim3(blocks, groups), dim3(64), 0, (hipStream_t)stream, G, parent, child,
+                         zlo, nlo, zhi, nhi);
+      return check_hip(hipGetLastError(), "prolongation column kernel launch");
+    }
   }
   const int64_t blocks = (total - r0 + 255) / 256;
   if (blocks == 0) return BSSN_OK;
