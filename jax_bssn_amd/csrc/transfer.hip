@@ -137,12 +137,12 @@ __global__ void __launch_bounds__(256) k_fmr_prolong(ProlongGeom G, const double
 // parent reads per point and component instead of 216.  The sums run in
 // the order of k_fmr_prolong (x innermost, then y, then z).
 constexpr int ROW_THREADS = 64;
-// A workgroup of the row and column kernels takes COMPS_PER_BLOCK of the
-// components (blockIdx.y picks which): one row or column group is a single
+// A workgroup of the row (column) kernel takes ROW_COMPS (COL_COMPS) of the
+// components, blockIdx.y picking which: one row or column group is a single
 // wave, and the split gives the launch enough waves to hide the latency of
-// the parent reads.
-constexpr int COMPS_PER_BLOCK = 3;
-static_assert(NCOMP % COMPS_PER_BLOCK == 0, "components split evenly");
+// the parent reads (measured best: three per row, one per column group).
+constexpr int ROW_COMPS = 3, COL_COMPS = 1;
+static_assert(NCOMP % ROW_COMPS == 0 && NCOMP % COL_COMPS == 0, "components split evenly");
 constexpr int ROW_MAXV = 1024;  // parent z nodes of one row (NZ / 2 + 7 at most)
 
 __global__ void __launch_bounds__(ROW_THREADS) k_fmr_prolong_rows(ProlongGeom G, const double *__restrict__ parent,
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(ROW_THREADS) k_fmr_prolong_rows(ProlongGeom G,
   const int64_t sx = (int64_t)G.parent.n[1] * G.parent.n[2], sy = G.parent.n[2];
   const int pn = G.parent.n[2];
   const int64_t crow = ((int64_t)i * NY + j) * NZ;
-  for (int c = (int)blockIdx.y * COMPS_PER_BLOCK; c < ((int)blockIdx.y + 1) * COMPS_PER_BLOCK; ++c) {
+  for (int c = (int)blockIdx.y * ROW_COMPS; c < ((int)blockIdx.y + 1) * ROW_COMPS; ++c) {
     const double *P = parent + c * G.parent.ncell;
     __syncthreads();  // every lane is done with the previous component's nodes
     for (int v = threadIdx.x; v < nv; v += ROW_THREADS) {
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(64) k_fmr_prolong_zcols(ProlongGeom G, const d
   const int vbase = lane < g ? first - zlo : nlo + first - zhi;
   const int64_t sx = (int64_t)G.parent.n[1] * G.parent.n[2], sy = G.parent.n[2];
   const int64_t id = ((int64_t)i * NY + j) * NZ + k;
-  for (int c = (int)blockIdx.y * COMPS_PER_BLOCK; c < ((int)blockIdx.y + 1) * COMPS_PER_BLOCK; ++c) {
+  for (int c = (int)blockIdx.y * COL_COMPS; c < ((int)blockIdx.y + 1) * COL_COMPS; ++c) {
     __syncthreads();  // every lane is done with the previous component's nodes
     if (node_lane) {
       const double *P = parent + c * G.parent.ncell;
@@ -354,9 +354,8 @@ int bssn_fmr_prolong(const bssn_grid *parent_grid, int32_t parent_ghost, const i
   const int rows = 2 * G.g * G.child.n[1] + G.active[0] * 2 * G.g;
   int64_t r0 = 0;
   if (rows > 0 && G.child.n[2] / 2 + 7 <= ROW_MAXV) {
-    const unsigned groups = NCOMP / COMPS_PER_BLOCK;
-    hipLaunchKernelGGL(k_fmr_prolong_rows, dim3((unsigned)rows, groups), dim3(ROW_THREADS), 0, (hipStream_t)stream, G,
-                       parent, child);
+    hipLaunchKernelGGL(k_fmr_prolong_rows, dim3((unsigned)rows, NCOMP / ROW_COMPS), dim3(ROW_THREADS), 0,
+                       (hipStream_t)stream, G, parent, child);
     if ((rc = check_hip(hipGetLastError(), "prolongation row kernel launch"))) return rc;
     r0 = (int64_t)rows * G.child.n[2];
     // first parent z node of child z index k (as lagrange6 forms it)
@@ -366,8 +365,8 @@ int bssn_fmr_prolong(const bssn_grid *parent_grid, int32_t parent_ghost, const i
     const int cols = G.active[0] * G.active[1];
     if (r0 + (int64_t)cols * 2 * G.g == total && 2 * G.g <= COL_LANES && nlo + nhi <= COL_LANES) {
       const unsigned blocks = (unsigned)((cols + COL_PER_BLOCK - 1) / COL_PER_BLOCK);
-      hipLaunchKernelGGL(k_fmr_prolong_zcols, dim3(blocks, groups), dim3(64), 0, (hipStream_t)stream, G, parent, child,
-                         zlo, nlo, zhi, nhi);
+      hipLaunchKernelGGL(k_fmr_prolong_zcols, dim3(blocks, NCOMP / COL_COMPS), dim3(64), 0, (hipStream_t)stream, G,
+                         parent, child, zlo, nlo, zhi, nhi);
       return check_hip(hipGetLastError(), "prolongation column kernel launch");
     }
   }
