@@ -2,7 +2,8 @@
 
 * prolongation / restriction kernels against the oracle's dense matrices
   and injection (oracle/fmr.py) on random data: 1e-13 relative (different
-  summation order only);
+  summation order only), for guard widths 1, 2, 4 and 7 (row, column and
+  point kernels);
 * one and two hierarchy steps against the oracle restatement of the frozen
   serial algorithm: per-field max-norm relative difference <= 1e-12 per
   level;
@@ -71,6 +72,36 @@ def test_prolongation_kernel_matches_oracle(J, offset, periodic):
     got = gc.cpu().numpy()
     err = np.abs(got - want).max() / np.abs(want).max()
     print("prolong", offset, periodic, f"{err:.2e}", flush=True)
+    assert err <= 1e-13
+    np.testing.assert_array_equal(got[act], child[act])
+
+
+@pytest.mark.parametrize("ghost,offset,periodic", [(1, 0, True), (2, 0, True), (7, 0, True), (2, 6, False),
+                                                   (7, 6, False)])
+def test_prolongation_ghost_widths(J, ghost, offset, periodic):
+    """Guard widths 1 and 2 take the column kernel for the z slabs; 7 needs
+    18 parent z nodes per column (more than its sixteen lanes) and takes the
+    point kernel."""
+    from jax_bssn_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(100 * ghost + offset)
+    spec = OF.FMRPatchSpec((2, 3, 2), (6, 6, 7), ghost_width=ghost)
+    pshape = tuple(n + 2 * offset for n in (12, 11, 13))
+    cshape = tuple(n + 2 * ghost for n in OF.fine_active_shape(spec))
+    parent = rng.standard_normal((21,) + pshape)
+    child = rng.standard_normal((21,) + cshape)
+    mats = OF.interpolation_matrices(pshape, spec, offset, periodic)
+    act = (slice(None),) + OF.fine_active_slice(spec)
+    want = OF.prolongate(parent, mats)
+    want[act] = child[act]
+    gp = torch.tensor(parent, device="cuda")
+    gc = torch.tensor(child, device="cuda")
+    print("prolong ghost", ghost, offset, periodic, pshape, cshape, flush=True)
+    _lib.check(lib.bssn_fmr_prolong(_grid(pshape), offset, _i3((int(periodic),) * 3), _p(gp), _grid(cshape),
+                                    _i3(spec.coarse_lo), ghost, _p(gc), None))
+    got = gc.cpu().numpy()
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("prolong ghost", ghost, f"{err:.2e}", flush=True)
     assert err <= 1e-13
     np.testing.assert_array_equal(got[act], child[act])
 
