@@ -75,14 +75,64 @@ __device__ __forceinline__ double axi_source(const AxiGeom &G, const double *__r
   return a + (f[(int64_t)(G.nx - 1) * G.nz + k] - a) * ratio_b[m - G.nx];
 }
 
+// Rotation of the radial profile by phi (cos cs, sin sn) about z, stored at
+// support point pos: v^i = L^i_a v^a, T_ij = L_ia L_jb T_ab with
+// L = [[c, -s, 0], [s, c, 0], [0, 0, 1]].  Every loop over components is
+// unrolled so that prof / out stay in registers (a runtime index would
+// place them in scratch memory).
+__device__ __forceinline__ void axi_rotate_store(const double prof[NCOMP], double cs, double sn,
+                                                 double *__restrict__ sup, int64_t nsup, int64_t pos) {
+  const double L[3][3] = {{cs, -sn, 0.0}, {sn, cs, 0.0}, {0.0, 0.0, 1.0}};
+  double out[NCOMP];
+#pragma unroll
+  for (int c = 0; c < NCOMP; ++c) out[c] = prof[c];
+  const int vecs[2] = {GAM, BETA};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int b = vecs[t];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[b + r] = L[r][0] * prof[b] + L[r][1] * prof[b + 1] + L[r][2] * prof[b + 2];
+  }
+  const int tens[2] = {GXX, AXX};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int b = tens[t];
+    double T[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) T[r][s] = prof[b + sym(r, s)];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int s = r; s < 3; ++s) {
+        double acc = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 3; ++bb) acc += L[r][a] * L[s][bb] * T[a][bb];
+        out[b + sym(r, s)] = acc;
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < NCOMP; ++c) sup[c * nsup + pos] = out[c];
+}
+
+// Planes j and 8 - j lie at y and -y: the same rho (so the same radial
+// profile) and opposite sin(phi).  One thread interpolates the profile once
+// for the pair p = 0..3 (planes p and 8 - p) or for the centre plane p = 4,
+// and stores each plane's rotation.
+constexpr int AX_PAIRS = AX_CENTER + 1;
+
 __global__ void __launch_bounds__(256, 2) k_axi_reconstruct(AxiGeom G, const double *__restrict__ u,
                                                          double *__restrict__ sup) {
   const int64_t nsup = (int64_t)G.nx * AX_SUPPORT * G.nz;
+  const int64_t nthr = (int64_t)G.nx * AX_PAIRS * G.nz;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nsup) return;
+  if (id >= nthr) return;
   const int k = (int)(id % G.nz);
-  const int j = (int)((id / G.nz) % AX_SUPPORT);
-  const int i = (int)(id / ((int64_t)G.nz * AX_SUPPORT));
+  const int j = (int)((id / G.nz) % AX_PAIRS);
+  const int i = (int)(id / ((int64_t)G.nz * AX_PAIRS));
   const double x = (i - 3.5) * G.dx, y = (j - 4.0) * G.dx;
   const double rho = sqrt(x * x + y * y);
   const bool on_axis = rho <= 0.0;
@@ -113,8 +163,6 @@ __global__ void __launch_bounds__(256, 2) k_axi_reconstruct(AxiGeom G, const dou
     w[a] = p;
   }
   const bool valid = q >= 0.0 && q <= (double)(size - 1);
-  // every loop over components is unrolled so that prof / out stay in
-  // registers (a runtime index would place them in scratch memory)
   double prof[NCOMP];
 #pragma unroll
   for (int c = 0; c < NCOMP; ++c) {
@@ -123,42 +171,11 @@ __global__ void __launch_bounds__(256, 2) k_axi_reconstruct(AxiGeom G, const dou
     for (int a = 0; a < 6; ++a) v += w[a] * axi_source(G, u, c, start + a, k, ratio);
     prof[c] = valid ? v : NAN;
   }
-  // rotate: v^i = L^i_a v^a, T_ij = L_ia L_jb T_ab with
-  // L = [[c, -s, 0], [s, c, 0], [0, 0, 1]]
-  const double L[3][3] = {{cs, -sn, 0.0}, {sn, cs, 0.0}, {0.0, 0.0, 1.0}};
-  double out[NCOMP];
-#pragma unroll
-  for (int c = 0; c < NCOMP; ++c) out[c] = prof[c];
-  const int vecs[3] = {GAM, BETA, -1};
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int b = vecs[t];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) out[b + r] = L[r][0] * prof[b] + L[r][1] * prof[b + 1] + L[r][2] * prof[b + 2];
-  }
-  const int tens[2] = {GXX, AXX};
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int b = tens[t];
-    double T[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int s = 0; s < 3; ++s) T[r][s] = prof[b + sym(r, s)];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int s = r; s < 3; ++s) {
-        double acc = 0.0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-          for (int bb = 0; bb < 3; ++bb) acc += L[r][a] * L[s][bb] * T[a][bb];
-        out[b + sym(r, s)] = acc;
-      }
-  }
-#pragma unroll
-  for (int c = 0; c < NCOMP; ++c) sup[c * nsup + id] = out[c];
+  axi_rotate_store(prof, cs, sn, sup, nsup, ((int64_t)i * AX_SUPPORT + j) * G.nz + k);
+  // the mirror plane 8 - j at -y: the same profile, sin(phi) of opposite
+  // sign (bitwise what the plane's own rho and y / rho give)
+  if (j < AX_CENTER)
+    axi_rotate_store(prof, cs, -sn, sup, nsup, ((int64_t)i * AX_SUPPORT + (2 * AX_CENTER - j)) * G.nz + k);
 }
 
 __global__ void __launch_bounds__(256) k_axi_project(AxiGeom G, const double *__restrict__ sup,
@@ -211,8 +228,8 @@ int bssn_cartoon_axi_reconstruct(const bssn_grid *compact, double dx, double z_m
   if (rc) return rc;
   if (!state || !support) return fail(BSSN_ERR_INVALID, "null argument");
   if (state == support) return fail(BSSN_ERR_INVALID, "state and support must not alias");
-  const int64_t nsup = (int64_t)G.nx * AX_SUPPORT * G.nz;
-  hipLaunchKernelGGL(k_axi_reconstruct, dim3((unsigned)((nsup + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+  const int64_t nthr = (int64_t)G.nx * AX_PAIRS * G.nz;
+  hipLaunchKernelGGL(k_axi_reconstruct, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      G, state, support);
   return check_hip(hipGetLastError(), "axisymmetric reconstruction kernel launch");
 }
