@@ -80,10 +80,12 @@ __global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
 
 // ------------------------------------------------------- region tiles
 
-// A box one plane thick along y (the Cartoon centre plane) as 8 x 1 x 32
-// tiles over [lo, hi), tile t at lo + ((t / nz) * 8, 0, (t % nz) * 32),
-// with closure rows along x and z (the box may end on those faces).
-constexpr int RTX = 8, RTY = 1, RTZ = 32;
+// A box one plane thick along y (the Cartoon centre plane) as 16 x 1 x 16
+// tiles over [lo, hi), tile t at lo + ((t / nz) * 16, 0, (t % nz) * 16),
+// with closure rows along x and z (the box may end on those faces).  The
+// square x-z footprint loads the fewest halo points per tile point (13.2
+// against 14.5 for 8 x 1 x 32 over the seven planes of the y halo).
+constexpr int RTX = 16, RTY = 1, RTZ = 16;
 
 struct RegionTiles {
   int lo[3], hi[3];

@@ -96,7 +96,7 @@ def test_compact_rhs_matches_oracle(C, z_somm, kappa):
 @pytest.mark.parametrize("kappa", [0.0, 0.02])
 def test_compact_rhs_on_region_tiles_matches_oracle(C, kappa):
     """full_nx = 32: 16 radial points after the 4 axis ghosts and 32 z
-    points, so the centre plane splits into 8 x 1 x 32 tiles that meet the
+    points, so the centre plane splits into 16 x 1 x 16 tiles that meet the
     Sommerfeld faces on tile boundaries (the region tile kernels of the RHS
     and of the kappa term; the other cases take the point kernels)."""
     v, p = case(full_nx=32, z_somm=True, kappa=kappa)
