@@ -93,17 +93,19 @@ def test_compact_rhs_matches_oracle(C, z_somm, kappa):
     assert max(errs.values()) <= 1e-10
 
 
-@pytest.mark.parametrize("kappa", [0.0, 0.02])
-def test_compact_rhs_on_region_tiles_matches_oracle(C, kappa):
+@pytest.mark.parametrize("full_nx,kappa", [(32, 0.0), (32, 0.02), (64, 0.02)])
+def test_compact_rhs_on_region_tiles_matches_oracle(C, full_nx, kappa):
     """full_nx = 32: 16 radial points after the 4 axis ghosts and 32 z
     points, so the centre plane splits into 16 x 1 x 16 tiles that meet the
     Sommerfeld faces on tile boundaries (the region tile kernels of the RHS
-    and of the kappa term; the other cases take the point kernels)."""
-    v, p = case(full_nx=32, z_somm=True, kappa=kappa)
+    and of the kappa term; the other cases take the point kernels); 64: 32
+    radial and 64 z points, 2 x 4 tiles, the middle z tiles away from the
+    faces."""
+    v, p = case(full_nx=full_nx, z_somm=True, kappa=kappa)
     got = C.compute_axisymmetric_rhs(_jv(v), _jp(p))
     want = OC.compute_axisymmetric_rhs(v, p)
     errs = rel_errors(got, want)
-    print("axi rhs region tiles", kappa, " ".join(f"{k}={e:.1e}" for k, e in errs.items()), flush=True)
+    print("axi rhs region tiles", full_nx, kappa, " ".join(f"{k}={e:.1e}" for k, e in errs.items()), flush=True)
     assert max(errs.values()) <= 1e-10
     got32 = C.compute_axisymmetric_rhs(_jv(v), _jp(p), precision="f32rhs")
     errs32 = rel_errors(got32, want)
