@@ -9,7 +9,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .time_evolve import _like, _ptr, stream_handle
+from .time_evolve import _like, _ptr, _same_grid, require_device_f64, require_packed, stream_handle
 from .variables import pack
 
 
@@ -28,8 +28,12 @@ NORM_NAMES = ("hamiltonian_l2", "momentum_l2", "det_gamma_l2", "trace_A_l2", "ga
 def constraints_packed(state, params, out=None, matter=None):
     """(9, nx, ny, nz): H, M_x, M_y, M_z, det-1, tr A, G^x, G^y, G^z."""
     lib = _lib.load()
+    require_packed(state)
+    if matter is not None:
+        _same_grid(matter, state, "matter", 10)
     if out is None:
         out = torch.empty((9,) + tuple(state.shape[1:]), dtype=torch.float64, device=state.device)
+    _same_grid(out, state, "out", 9)
     _lib.check(lib.bssn_constraints_with_matter(
         ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(state.shape[1:])), _ptr(state),
         _ptr(matter) if matter is not None else None, _ptr(out), stream_handle()))
@@ -39,6 +43,7 @@ def constraints_packed(state, params, out=None, matter=None):
 def norms_packed(cfields):
     """Device tensor of the 10 norms of a (9, nx, ny, nz) constraint tensor."""
     lib = _lib.load()
+    require_device_f64(cfields, "constraint fields", ncomp=9, ndim=4)
     grid = _lib.c_grid(cfields.shape[1:])
     partial = torch.empty(int(lib.bssn_norm_partial_size(ctypes.byref(grid))), dtype=torch.float64,
                           device=cfields.device)

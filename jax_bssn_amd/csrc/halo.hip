@@ -144,6 +144,19 @@ int bssn_tiled_ghost_fill(const bssn_tile_layout *layout, const int32_t *bc, int
     if (bc[k] != 0 && bc[k] != 1) return fail(BSSN_ERR_INVALID, "boundary codes must be 0 (periodic) or 1 (Sommerfeld)");
     H.bc[k] = bc[k];
   }
+  // The linear exterior rule takes its slope from the face sample and the
+  // owned sample next to it; a face tile owning a single sample would read a
+  // guard slot that the same launch writes.
+  if (exterior == 0) {
+    for (int a = 0; a < 3; ++a) {
+      const int owned_lo = H.width[a] + (H.counts[a] == 1 ? H.endpoint[a] : 0);
+      const int owned_hi = H.width[a] + H.endpoint[a];
+      if ((H.bc[2 * a] == 1 && owned_lo < 2) || (H.bc[2 * a + 1] == 1 && owned_hi < 2))
+        return fail(BSSN_ERR_INVALID,
+                    "linear extrapolation on a physical face of axis %d needs at least two owned samples per face tile",
+                    a);
+    }
+  }
   if (tile_size * ntiles * ncomp >= ((int64_t)1 << 40)) return fail(BSSN_ERR_INVALID, "tiled field too large");
   H.tile_size = tile_size;
   for (int a = 0; a < 3; ++a) {

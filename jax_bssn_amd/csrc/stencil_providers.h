@@ -19,10 +19,16 @@
 
 #include "bssn_common.h"
 
+// Components whose LDS boxes are held in registers ahead of their visit.
+#ifndef BSSN_PREFETCH_DEPTH
+#define BSSN_PREFETCH_DEPTH 1
+#endif
+
 namespace bssn {
 
 // Visit order of the components in the per-point RHS (each visit may
-// restage an LDS tile; the tile provider prefetches visit v + 1 during v).
+// restage an LDS tile; the tile provider prefetches visits v + 1 .. v + PD
+// during v).
 __host__ __device__ constexpr int visit_comp(int v) {
   // shift 0..2, metric 3..8, lapse 9, W 10, K 11, connection 12..14, A 15..20
   return v < 3 ? BETA + v
@@ -262,7 +268,10 @@ struct TileProvider {
   int64_t ncell;
   int goff[NSLOT];
   int boff;       // tile origin (the FP32 offset of each component)
-  T pref[NSLOT];  // register prefetch of the next component's tile
+  // register prefetch of the next PD components' tiles (ring indexed by the
+  // visit number, a compile-time constant in the unrolled per-point code)
+  static constexpr int PD = BSSN_PREFETCH_DEPTH;
+  T pref[PD][NSLOT];
   int c;
   int tid;
   T inv_dx, inv_dx2, ko_coef;
@@ -282,11 +291,11 @@ struct TileProvider {
   // tile origin (one block-uniform FP64 load): every stencil's weights sum
   // to zero, so derivatives are unchanged, and FP32 keeps the digits of the
   // variation instead of those of a large constant background.
-  __device__ __forceinline__ void fetch(int comp) {
+  __device__ __forceinline__ void fetch(int comp, int slot) {
     const BSSN_GLOBAL double *f = src + comp * ncell;
     const double base = sizeof(T) == sizeof(double) ? 0.0 : f[boff];
 #pragma unroll
-    for (int k = 0; k < NSLOT; ++k) pref[k] = (T)(f[goff[k]] - base);
+    for (int k = 0; k < NSLOT; ++k) pref[slot][k] = (T)(f[goff[k]] - base);
   }
 
   __device__ __forceinline__ void setup(const Ctx &C, const double *u, T *smem, double *stash_mem, int x0, int y0, int z0,
@@ -337,7 +346,10 @@ struct TileProvider {
       goff[k] = gx * C.ax[0].stride + gy * C.ax[1].stride + gz;
     }
     boff = x0 * C.ax[0].stride + y0 * C.ax[1].stride + z0;
-    if (prefetch) fetch(visit_comp(0));
+    if (prefetch) {
+#pragma unroll
+      for (int d = 0; d < PD; ++d) fetch(visit_comp(d), d);
+    }
   }
 
   // Stage one component from an arbitrary array (component base f) into
@@ -347,31 +359,33 @@ struct TileProvider {
   __device__ __forceinline__ void stage_from(const double *f) {
     const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)f;
 #pragma unroll
-    for (int k = 0; k < NSLOT; ++k) pref[k] = (T)g[goff[k]];
+    for (int k = 0; k < NSLOT; ++k) pref[0][k] = (T)g[goff[k]];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;
-      if (k < NSLOT - 1 || e < NBOX) lds[e] = pref[k];
+      if (k < NSLOT - 1 || e < NBOX) lds[e] = pref[0][k];
     }
     __syncthreads();
     cur = lds;
   }
 
-  // Stage visit v: store the prefetched tile, one barrier, prefetch v + 1.
-  // Double-buffered, buffer v & 1 was last read in visit v - 2, which every
-  // thread finished before arriving at the barrier of visit v - 1; a single
-  // buffer needs a second barrier before it is overwritten.
+  // Stage visit v: store the prefetched tile, one barrier, prefetch v + PD
+  // into the ring slot just emptied.  Double-buffered, buffer v & 1 was last
+  // read in visit v - 2, which every thread finished before arriving at the
+  // barrier of visit v - 1; a single buffer needs a second barrier before it
+  // is overwritten.
   __device__ __forceinline__ void stage(int v) {
     BSSN_LDS T *buf = lds + (NBUF == 2 ? (v & 1) * NBOX : 0);
+    const int slot = v % PD;
     if (NBUF == 1 && v > 0) __syncthreads();  // every thread is done with visit v - 1
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;
-      if (k < NSLOT - 1 || e < NBOX) buf[e] = pref[k];
+      if (k < NSLOT - 1 || e < NBOX) buf[e] = pref[slot][k];
     }
     __syncthreads();
-    if (v + 1 < NCOMP) fetch(visit_comp(v + 1));
+    if (v + PD < NCOMP) fetch(visit_comp(v + PD), slot);
     cur = buf;
   }
 

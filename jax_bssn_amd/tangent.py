@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .time_evolve import _ptr, stream_handle
+from .time_evolve import _ptr, require_packed, stream_handle
 from .variables import NCOMP, device
 
 
@@ -74,6 +74,7 @@ def hamiltonian_l2_tangent(state, dstate, params):
     """(L, dL) with L = sqrt(mean H^2) over all points and dL its derivative
     along each direction of dstate (mean(H dH) / L); dL is a float for one
     direction and a list for a stack."""
+    require_packed(state)
     d, single = _dirs(state, dstate)
     shape = tuple(state.shape[1:])
     H = torch.empty(shape, dtype=torch.float64, device=state.device)

@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from .constraints import NORM_NAMES
-from .time_evolve import _ptr, precision_code, stream_handle
+from .time_evolve import _ptr, precision_code, require_device_f64, stream_handle
 from .variables import NCOMP, BSSNVariables, device
 
 FACE_NAMES = ("xl_bc", "xr_bc", "yl_bc", "yr_bc", "zl_bc", "zr_bc")
@@ -116,6 +116,10 @@ def refresh_tiled_field(field, params, layout, exterior="linear"):
     """In-place ghost fill of a tile-major device field (ntiles, ncomp, *alloc)."""
     if exterior not in EXTERIOR:
         raise ValueError("Unknown physical exterior rule")
+    require_device_f64(field, "tiled field")
+    want = (layout.ntiles,) + tuple(layout.allocated_shape)
+    if field.dim() != 5 or field.shape[0] != want[0] or tuple(field.shape[2:]) != want[1:]:
+        raise ValueError(f"tiled field has shape {tuple(field.shape)}; expected ({want[0]}, ncomp) + {want[1:]}")
     cl = layout.c_layout()
     _lib.check(_lib.load().bssn_tiled_ghost_fill(ctypes.byref(cl), _bcs(params), EXTERIOR[exterior],
                                                  int(field.shape[1]), _ptr(field), stream_handle()))

@@ -33,6 +33,38 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def require_device_f64(t, name, ncomp=None, ndim=None):
+    """Raise ValueError unless ``t`` is a contiguous float64 tensor on a HIP
+    device (with ``ndim`` dimensions and ``ncomp`` leading components when
+    given): the C ABI takes bare device pointers, so anything else would reach
+    the kernels as a host address or a too-small buffer."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor on the GPU, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live on a HIP device, not {t.device}")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name} must be float64, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if ndim is not None and t.dim() != ndim:
+        raise ValueError(f"{name} must have {ndim} dimensions, got shape {tuple(t.shape)}")
+    if ncomp is not None and (t.dim() < 1 or t.shape[0] != ncomp):
+        raise ValueError(f"{name} must have {ncomp} leading components, got shape {tuple(t.shape)}")
+    return t
+
+
+def require_packed(t, name="state"):
+    """A packed (21, nx, ny, nz) FP64 device state (see require_device_f64)."""
+    return require_device_f64(t, name, ncomp=NCOMP, ndim=4)
+
+
+def _same_grid(t, state, name, ncomp):
+    require_device_f64(t, name, ncomp=ncomp, ndim=4)
+    if tuple(t.shape[1:]) != tuple(state.shape[1:]) or t.device != state.device:
+        raise ValueError(f"{name} has shape {tuple(t.shape)} on {t.device}; expected "
+                         f"({ncomp},) + {tuple(state.shape[1:])} on {state.device}")
+
+
 def _like(v):
     return v.conformal_factor
 
@@ -45,9 +77,11 @@ def _scratch(params, shape, what, device):
 def rhs_packed(state, params, precision="f64", out=None):
     """RHS of a packed (21, nx, ny, nz) device state."""
     lib = _lib.load()
+    require_packed(state)
     shape = tuple(state.shape[1:])
     if out is None:
         out = torch.empty_like(state)
+    _same_grid(out, state, "out", NCOMP)
     work = _scratch(params, shape, 0, state.device)
     _lib.check(lib.bssn_rhs(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(shape)),
                             precision_code(precision), _ptr(state), _ptr(out), _ptr(work), stream_handle()))
@@ -71,9 +105,12 @@ def pack_matter(energy_density, momentum_density, spatial_stress, device_):
 
 def rhs_with_matter_packed(state, matter, params, precision="f64", out=None):
     lib = _lib.load()
+    require_packed(state)
+    _same_grid(matter, state, "matter", 10)
     shape = tuple(state.shape[1:])
     if out is None:
         out = torch.empty_like(state)
+    _same_grid(out, state, "out", NCOMP)
     work = _scratch(params, shape, 0, state.device)
     _lib.check(lib.bssn_rhs_with_matter(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(shape)),
                                         precision_code(precision), _ptr(state), _ptr(matter), _ptr(out),
@@ -83,8 +120,10 @@ def rhs_with_matter_packed(state, matter, params, precision="f64", out=None):
 
 def project_packed(state, out=None):
     lib = _lib.load()
+    require_packed(state)
     if out is None:
         out = torch.empty_like(state)
+    _same_grid(out, state, "out", NCOMP)
     _lib.check(lib.bssn_enforce_constraints(ctypes.byref(_lib.c_grid(state.shape[1:])), _ptr(state),
                                             _ptr(out), stream_handle()))
     return out
@@ -93,8 +132,14 @@ def project_packed(state, out=None):
 def evolve_packed(state, params, nsteps, work=None, precision="f64"):
     """rk4_step applied nsteps times to a packed device state, in place."""
     lib = _lib.load()
+    require_packed(state)
+    need = _lib.work_size(params, tuple(state.shape[1:]), 1)
     if work is None:
         work = _scratch(params, tuple(state.shape[1:]), 1, state.device)
+    require_device_f64(work, "work")
+    if work.numel() < need or work.device != state.device:
+        raise ValueError(f"work holds {work.numel()} doubles on {work.device}; bssn_evolve needs {need} "
+                         f"on {state.device}")
     _lib.check(lib.bssn_evolve(ctypes.byref(_lib.c_params(params)), ctypes.byref(_lib.c_grid(state.shape[1:])),
                                precision_code(precision), int(nsteps), _ptr(state), _ptr(work),
                                stream_handle()))
@@ -145,7 +190,7 @@ class Evolver:
         self.params = params
         self.precision = precision
         if isinstance(vars_or_state, torch.Tensor):
-            self.state = vars_or_state
+            self.state = require_packed(vars_or_state)
         else:
             self.state = pack(vars_or_state)
         self.shape = tuple(self.state.shape[1:])
@@ -166,6 +211,9 @@ class Evolver:
 
     def stage(self, s, u_s, u0, ustar, u_next, reproject=False):
         lib = _lib.load()
+        for name, t in (("u_s", u_s), ("u0", u0), ("ustar", ustar), ("u_next", u_next)):
+            if t is not None:
+                _same_grid(t, self.state, name, NCOMP)
         _lib.check(lib.bssn_rk4_stage(ctypes.byref(self._cp), ctypes.byref(self._cg), self._prec, int(s),
                                       _ptr(u_s), _ptr(u0) if u0 is not None else None, _ptr(ustar),
                                       _ptr(u_next), 1 if reproject else 0, _ptr(self.stage_scratch),

@@ -9,6 +9,7 @@ tensor algebra so that the two can be compared.
 * periodic_coordinate_arrays     tests/initial_data.py:62-73
 * periodic_gauge_wave_state      tests/initial_data.py:76-144
 * brill_lindquist_conformal_factor  utilities/bowen_york_solver.py:52-63
+* linear_wave_data               tests/initial_data.py:17-59, 208-280
 """
 
 import numpy as np
@@ -107,3 +108,40 @@ def puncture_state(X, Y, Z, masses=(1.0,), positions=((0.0, 0.0, 0.0),),
     z3 = np.zeros((3,) + shape)
     return BSSNVariables(gamma, W, np.zeros((3, 3) + shape), np.zeros(shape), z3.copy(),
                          lapse, z3.copy())
+
+
+def vertex_coordinate_arrays(ni, nj, nk, dx):
+    """Vertex-centred coordinates (i - (n - 1) / 2) dx (tests/initial_data.py:17-38)."""
+    axes = [(np.arange(n) - (n - 1) / 2.0) * dx for n in (ni, nj, nk)]
+    return np.meshgrid(*axes, indexing='ij')
+
+
+def conformal_connection_from_metric(metric, dx, mad_q=1.0):
+    """Gamma^i = gamma^mn Gamma^i_mn with periodic D1 of the metric
+    (tests/initial_data.py:41-59)."""
+    derivs = np.stack([diff1_field(metric, d + 2, dx, mad_q=mad_q) for d in range(3)])
+    inv = invert_3x3_metric(metric)
+    return np.einsum('mn...,imn...->i...', inv, christoffel_symbols_second_kind(inv, derivs))
+
+
+def linear_wave_data(ni, nj, nk, dx, amplitude=1.0e-8, wavelength=1.0, mad_q=1.0):
+    """Plus-polarised linear wave in Gauss coordinates (tests/initial_data.py:208-280):
+    gamma_yy = 1 + b, gamma_zz = 1 - b, b = A sin(2 pi x / d), W = (1 - b^2)^(-1/6),
+    K_yy = -K_zz = -1/2 db/dt, Gamma^i from the discrete metric derivatives."""
+    shape = (ni, nj, nk)
+    X, _, _ = vertex_coordinate_arrays(ni, nj, nk, dx)
+    b0 = amplitude * np.sin(2.0 * np.pi * X / wavelength)
+    W = np.power(1.0 - b0 ** 2, -1.0 / 6.0)
+    gamma = np.zeros((3, 3) + shape)
+    gamma[0, 0] = W ** 2
+    gamma[1, 1] = (1.0 + b0) * W ** 2
+    gamma[2, 2] = (1.0 - b0) * W ** 2
+    dbdt = -(2.0 * np.pi * amplitude / wavelength) * np.cos(2.0 * np.pi * X / wavelength)
+    Kij = np.zeros_like(gamma)
+    Kij[1, 1] = -0.5 * dbdt
+    Kij[2, 2] = 0.5 * dbdt
+    inv = invert_3x3_metric(gamma)
+    A = W ** 2 * traceless_part(Kij, gamma, inv)
+    K = W ** 2 * trace_tensor(Kij, inv)
+    Gam = conformal_connection_from_metric(gamma, dx, mad_q)
+    return BSSNVariables(gamma, W, A, K, Gam, np.ones(shape), np.zeros((3,) + shape))

@@ -123,3 +123,48 @@ def test_tangent_argument_validation(lib):
     pk = ctypes.byref(_lib.c_params(BSSNParameters(dx=0.1, kappa=0.02)))
     with pytest.raises(NotImplementedError, match="kappa"):
         _lib.check(lib.bssn_evolve_tangent(pk, g, 1, 1, buf, buf, buf, None))
+
+
+def test_ghost_fill_rejects_single_sample_linear_faces(lib):
+    """A face tile owning one sample has no owned inward neighbour for the
+    linear exterior slope; the fill refuses it before any launch."""
+    thin = _lib.CTileLayout((ctypes.c_int32 * 3)(1, 2, 2), (ctypes.c_int32 * 3)(1, 4, 4), 1,
+                            (ctypes.c_int32 * 3)(0, 0, 0))
+    bcs = (ctypes.c_int32 * 6)(1, 1, 0, 0, 0, 0)
+    buf = ctypes.c_void_p(0x1000)
+    with pytest.raises(ValueError, match="two owned samples"):
+        _lib.check(lib.bssn_tiled_ghost_fill(ctypes.byref(thin), bcs, 0, 1, buf, None))
+    # the constant and preserve rules read only the face sample: accepted up
+    # to the launch (not made here: the buffer is not a device pointer)
+    periodic = (ctypes.c_int32 * 6)(0, 0, 0, 0, 0, 0)
+    ok = _lib.CTileLayout((ctypes.c_int32 * 3)(1, 2, 2), (ctypes.c_int32 * 3)(2, 4, 4), 1,
+                          (ctypes.c_int32 * 3)(0, 0, 0))
+    with pytest.raises(ValueError, match="exterior"):
+        _lib.check(lib.bssn_tiled_ghost_fill(ctypes.byref(ok), periodic, 3, 1, buf, None))
+
+
+def test_packed_entry_points_reject_unsuitable_tensors():
+    """The Python mirror hands bare pointers to the C ABI, so every packed
+    entry point checks device, dtype, contiguity and shape first."""
+    import torch
+
+    from jax_bssn_amd import constraints as Cn
+    from jax_bssn_amd import time_evolve as T
+    from jax_bssn_amd.variables import BSSNParameters
+    p = BSSNParameters(dx=0.1)
+    host = torch.zeros((21, 8, 8, 8), dtype=torch.float64)
+    with pytest.raises(ValueError, match="HIP device"):
+        T.rhs_packed(host, p)
+    with pytest.raises(ValueError, match="HIP device"):
+        T.evolve_packed(host, p, 1)
+    with pytest.raises(ValueError, match="HIP device"):
+        T.project_packed(host)
+    with pytest.raises(ValueError, match="HIP device"):
+        Cn.constraints_packed(host, p)
+    with pytest.raises(ValueError, match="HIP device"):
+        T.Evolver(host, p)
+    with pytest.raises(ValueError, match="torch tensor"):
+        T.require_packed(host.numpy())
+    meta = torch.empty((21, 8, 8, 8), dtype=torch.float64, device="meta")
+    with pytest.raises(ValueError, match="HIP device"):
+        T.require_packed(meta)
