@@ -200,16 +200,17 @@ def fmr_linear_wave(J, steps, n=64, levels=3, amplitude=1.0e-8):
     base = J.BSSNParameters(eta=0.0, kappa=0.0, nu=0.0, g=0.0, dx=dx, dt=dt, zero_shift=1, gauge=0,
                             x_min=origin, y_min=origin, z_min=origin)
     patches, active, params = [], [(n,) * 3], [base]
-    for _ in range(1, levels):
+    for lv in range(1, levels):
         parent = params[-1]
+        pg = 0 if lv == 1 else 4  # guard points before the parent's active vertex 0
         lo = tuple((s - s // 2) // 2 for s in active[-1])
         hi = tuple(l + s // 2 - 1 for l, s in zip(lo, active[-1]))
         patches.append(F.FMRPatchSpec(lo, hi))
         active.append(tuple(2 * (h - l) + 1 for l, h in zip(lo, hi)))
         cdx = parent.dx / 2
-        params.append(parent._replace(dx=cdx, x_min=parent.x_min + lo[0] * parent.dx - 4 * cdx,
-                                      y_min=parent.y_min + lo[1] * parent.dx - 4 * cdx,
-                                      z_min=parent.z_min + lo[2] * parent.dx - 4 * cdx))
+        params.append(parent._replace(dx=cdx, x_min=parent.x_min + (pg + lo[0]) * parent.dx - 4 * cdx,
+                                      y_min=parent.y_min + (pg + lo[1]) * parent.dx - 4 * cdx,
+                                      z_min=parent.z_min + (pg + lo[2]) * parent.dx - 4 * cdx))
     states = []
     for lv, p in enumerate(params):
         shape = active[lv] if lv == 0 else tuple(a + 8 for a in active[lv])
@@ -226,13 +227,21 @@ def fmr_linear_wave(J, steps, n=64, levels=3, amplitude=1.0e-8):
     torch.cuda.synchronize()
     t = e0.elapsed_time(e1) / 1e3
     cells = sum(int(np.prod(a)) for a in active)
-    h = ev.u[-1][1][4:-4, 4:-4, 4:-4]  # gamma~_xy stays zero; the finest level's gamma~_yy - gamma~_zz is 2h
-    dh = (ev.u[-1][3] - ev.u[-1][5])[4:-4, 4:-4, 4:-4] / 2
+    # h+ = (gamma~_yy - gamma~_zz) / (2 W^2) on the active points of every
+    # level against the plane wave A sin 2 pi (x - t), relative to A
+    time_now = (steps + 1) * dt
+    errors = []
+    for lv, (u, p) in enumerate(zip(ev.u, params)):
+        g = 0 if lv == 0 else 4
+        a = u[:, g:u.shape[1] - g, g:u.shape[2] - g, g:u.shape[3] - g]
+        hp = 0.5 * (a[3] - a[5]) / a[6] ** 2
+        x = p.x_min + p.dx * (g + torch.arange(a.shape[1], dtype=torch.float64, device=a.device))
+        exact = amplitude * torch.sin(2 * np.pi * (x - time_now))[:, None, None]
+        errors.append(float((hp - exact).abs().max()) / amplitude)
     return {"value": cells * steps / t, "unit": "cell-updates/s (active points, all levels)",
             "levels": levels, "active_shapes": [list(a) for a in active], "ms_per_step": 1e3 * t / steps,
-            "steps": steps, "finite": bool(all(torch.isfinite(u).all() for u in ev.u)),
-            "finest_h_plus_max_over_amplitude": float(dh.abs().max()) / amplitude,
-            "finest_gamma_xy_max": float(h.abs().max())}
+            "steps": steps + 1, "finite": bool(all(torch.isfinite(u).all() for u in ev.u)),
+            "h_plus_max_error_over_amplitude_by_level": errors}
 
 
 def axisymmetric_puncture(J, steps, nr=512, nz=1024, separation=1.5, precision="f64", keep_state=False):
@@ -351,28 +360,53 @@ def copy_calibration(nbytes=1 << 31, reps=3):
     del src, dst
 
 
-def cpu_baseline(n=48, steps=1):
-    """FP64 NumPy oracle on the host: same puncture configuration at n^3."""
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(sizes=(32, 48, 64), steps=3):
+    """FP64 NumPy oracle on the host, one thread: the same puncture
+    configuration at each size, one warm-up step and then the median of
+    ``steps`` timed RK4 steps.  ``value`` is the rate at the largest size."""
     import numpy as np
     from threadpoolctl import threadpool_limits
 
     from oracle import bssn as OB
     from oracle import initial_data as OI
     L = 8.0
-    dx = 2 * L / n
-    x = -L + (np.arange(n) + 0.5) * dx
-    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
-    v = OI.puncture_state(X, Y, Z, masses=(1.0,), positions=((0.0, 0.0, 0.0),))
-    p = puncture_params(OB.BSSNParameters, dx, x[0])
+    per_size = {}
     with threadpool_limits(limits=1):
-        v = OB.rk4_step(v, p)  # warm-up
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            v = OB.rk4_step(v, p)
-        dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} RK4 step(s) of the same puncture configuration at {n}^3 "
-                      f"(dx = {dx}), NumPy FP64 oracle, 1 thread, after 1 warm-up step"}
+        for n in sizes:
+            dx = 2 * L / n
+            x = -L + (np.arange(n) + 0.5) * dx
+            X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+            v = OI.puncture_state(X, Y, Z, masses=(1.0,), positions=((0.0, 0.0, 0.0),))
+            p = puncture_params(OB.BSSNParameters, dx, x[0])
+            v = OB.rk4_step(v, p)  # warm-up
+            times = []
+            for _ in range(steps):
+                t0 = time.perf_counter()
+                v = OB.rk4_step(v, p)
+                times.append(time.perf_counter() - t0)
+            med = float(np.median(times))
+            per_size[str(n)] = {"median_step_seconds": med, "cell_updates_per_s": n ** 3 / med}
+            print(f"[cpu baseline] {n}^3: median step {med:.2f} s", file=sys.stderr, flush=True)
+    n = max(sizes)
+    threads = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    return {"value": per_size[str(n)]["cell_updates_per_s"], "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"median of {steps} RK4 steps (after 1 warm-up) of the same puncture configuration at "
+                      f"{', '.join(f'{m}^3' for m in sizes)} on [-8, 8]^3; value at {n}^3; NumPy FP64 oracle "
+                      "limited to 1 thread (threadpoolctl)",
+            "per_size": per_size, "cpu_model": _cpu_model(), "host_cores": len(os.sched_getaffinity(0)),
+            "numpy": np.__version__, "thread_env": threads}
 
 
 def main():

@@ -2,8 +2,9 @@
 wave (A = 0.1, d = 1, [-0.5, 0.5)^3, harmonic lapse, zero shift, KO
 nu = 0.1, dt = dx / 4).
 
-* per-field max-norm difference against the FP64 oracle after every step
-  of a 32^3 run (<= 1e-12, the per-step bar);
+* per-field max-norm difference against the FP64 oracle for every one of
+  the configuration's 200 steps at 32^3 (<= 1e-12 per step from the
+  oracle's state; <= 1e-11 for the free-running evolution);
 * the L2 Hamiltonian constraint at the common time T = 1.5625 (200, 400 and
   800 steps at 32^3, 64^3, 128^3) converges at fourth order: observed
   orders above the reference's 3.8 (tests/test_gauge_wave.py,
@@ -37,21 +38,45 @@ def _params(n):
 
 
 def test_gauge_wave_matches_oracle_every_step(J):
-    n = 32
+    """All 200 steps of the configuration at 32^3.  The gauge wave depends on
+    x only, so the oracle steps the 32 x 1 x 1 column with y and z reduced
+    (their derivatives vanish identically, as they do to roundoff on the
+    cube); every (y, z) column of the GPU state is compared with it.
+
+    Two bars: each GPU step taken from the oracle's state agrees with the
+    oracle's step to 1e-12 per field (the per-step bar); the free-running
+    GPU evolution stays within 1e-11 of the oracle's over the 200 steps (the
+    two accumulate independent roundoff, which grows with the step count)."""
+    n, steps = 32, 200
     X, Y, Z = OI.periodic_coordinate_arrays(n)
     v = OI.gauge_wave_analytic_state(X, Y, Z)
     p = _params(n)
+    col = OB.BSSNVariables(*(np.ascontiguousarray(f[..., :1, :1]) for f in v))
+    pc = p._replace(reduced_axes=(False, True, True))
     from jax_bssn_amd.variables import pack, unpack
-    ev = J.Evolver(pack(v), J.BSSNParameters(**p._asdict()))
-    ref = v
-    worst = 0.0
-    for step in range(12):
-        ref = OB.rk4_step(ref, p)
-        ev.step(1)
-        errs = rel_errors(unpack(ev.state, np.zeros(1)), ref)
-        worst = max(worst, max(errs.values()))
-        assert max(errs.values()) <= 1e-12, (step, errs)
-    print("gauge wave 32^3, 12 steps, worst per-field difference", f"{worst:.2e}", flush=True)
+    jp = J.BSSNParameters(**p._asdict())
+    free = J.Evolver(pack(v), jp)
+    one = J.Evolver(pack(v), jp)
+
+    def cube(c):
+        return OB.BSSNVariables(*(np.broadcast_to(f, f.shape[:-2] + (n, n)) for f in c))
+    ref = col
+    worst_step = worst_free = 0.0
+    for step in range(steps):
+        prev = ref
+        ref = OB.rk4_step(ref, pc)
+        one.state.copy_(pack(cube(prev)))
+        one.step(1)
+        free.step(1)
+        e_step = rel_errors(unpack(one.state, np.zeros(1)), cube(ref))
+        e_free = rel_errors(unpack(free.state, np.zeros(1)), cube(ref))
+        worst_step = max(worst_step, max(e_step.values()))
+        worst_free = max(worst_free, max(e_free.values()))
+        assert max(e_step.values()) <= 1e-12, (step, e_step)
+        assert max(e_free.values()) <= 1e-11, (step, e_free)
+        if (step + 1) % 50 == 0:
+            print(f"gauge wave 32^3 step {step + 1}: worst per-step difference {worst_step:.2e}, "
+                  f"free-running {worst_free:.2e}", flush=True)
 
 
 def _hamiltonian_l2(J, n, precision):
