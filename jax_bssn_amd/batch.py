@@ -113,7 +113,7 @@ class BatchEvolver:
         cus = _cu_count()
         best, best_t = (0, 2 * cus), float("inf")
         for v in TILE_VARIANTS:
-            for blocks in (2 * cus, 4 * cus, 1 << 20):
+            for blocks in (cus, 2 * cus, 4 * cus, 1 << 20):
                 self.prepare(v, blocks)
                 self._step_once()
                 torch.cuda.synchronize()

@@ -28,10 +28,6 @@
 #include "stage_common.h"
 #include "stencil_providers.h"
 
-#ifndef BSSN_MIN_WAVES
-#define BSSN_MIN_WAVES 2
-#endif
-
 namespace bssn {
 namespace {
 
@@ -55,10 +51,12 @@ __host__ __device__ constexpr int variant_tile(int v, int a) {
   return v == 0 ? (a == 0 ? 4 : 8) : (v == 1 ? (a == 2 ? 4 : 8) : (a == 1 ? 4 : 8));
 }
 
+// Compiled for the occupancy of the single-grid interior tile kernel (one
+// wave per SIMD in FP64, two in FP32).
 template <typename T, int TX, int TY, int TZ>
-__global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
+__global__ void __launch_bounds__(TX *TY *TZ, (sizeof(T) == sizeof(double) ? 1 : 2))
     k_batch_tile(const BatchDesc *__restrict__ D, const int *__restrict__ prefix, int ninst, int total) {
-  using P = TileProvider<T, TX, TY, TZ, false, false>;
+  using P = TileProvider<T, TX, TY, TZ, false, false, 7, sizeof(T) == sizeof(double)>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   const int tid = threadIdx.x;

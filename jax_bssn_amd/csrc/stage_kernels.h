@@ -16,10 +16,16 @@
 #include "stencil_providers.h"
 #include "tiles.h"
 
-// Minimum waves per SIMD the tile kernels are compiled for (caps VGPRs).
-#ifndef BSSN_MIN_WAVES
-#define BSSN_MIN_WAVES 2
-#endif
+// Minimum waves per SIMD a tile kernel is compiled for (caps its registers
+// at 512 / waves per lane, VGPRs and AGPRs together).  The FP64 kernels
+// without closure rows hold their whole point algebra in registers at one
+// wave per SIMD (about 500, no scratch); at two they spill 600-1000 B per
+// lane and run 12-40 % slower (256^3 puncture, MI355X).  The closure
+// kernels and the FP32 ones run best at two.
+template <typename T, bool CL>
+constexpr int tile_min_waves() {
+  return (sizeof(T) == sizeof(double) && !CL) ? 1 : 2;
+}
 
 namespace bssn {
 
@@ -42,9 +48,9 @@ __device__ __forceinline__ void load_point(const Ctx &C, const double *__restric
 // ------------------------------------------------------------ tile kernel
 
 template <typename T, int TX, int TY, int TZ, bool MAD, bool CL, int CLA = 7>
-__global__ void __launch_bounds__(TX *TY *TZ, BSSN_MIN_WAVES)
+__global__ void __launch_bounds__(TX *TY *TZ, (tile_min_waves<T, CL>()))
     k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, TileSet S) {
-  using P = TileProvider<T, TX, TY, TZ, MAD, CL, CLA>;
+  using P = TileProvider<T, TX, TY, TZ, MAD, CL, CLA, sizeof(T) == sizeof(double) && !CL>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   int bx, by, bz, slab;
@@ -119,7 +125,7 @@ inline bool region_tiled(const Ctx &C, const int *lo, const int *hi) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(RTX *RTY *RTZ, BSSN_MIN_WAVES)
+__global__ void __launch_bounds__(RTX *RTY *RTZ, (tile_min_waves<T, true>()))
     k_stage_region(Ctx C, const double *__restrict__ us, StageOut O, RegionTiles R) {
   using P = TileProvider<T, RTX, RTY, RTZ, false, true, 5>;
   __shared__ T smem[P::LDS_ELEMS];
