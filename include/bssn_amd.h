@@ -144,6 +144,18 @@ int64_t bssn_norm_partial_size(const bssn_grid *grid);
 int bssn_constraint_norms(const bssn_grid *grid, const double *constraints,
                           double *partial, double *norms, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  bssn_constraint_norms with the reduction's partial sums in a
+ * static device array of the library (1024 x 10 doubles = 80 KB on each
+ * device it runs on, held until the process ends) instead of a caller
+ * buffer; calls no allocating routine.  Grids of at most 2^31 - 1 points per
+ * component (BSSN_ERR_INVALID beyond).  Every call on one device shares that
+ * device's array: calls that overlap on one device (concurrent streams)
+ * race. */
+int bssn_constraint_norms_noalloc(const bssn_grid *grid, const double *constraints,
+                                  double *norms, void *stream);
+
 /* Fixed mesh refinement (vertex-centred, 2:1).  A child patch with parent
  * active bounds lo..hi (inclusive, per axis) is allocated with
  * 2*(hi-lo)+1 active vertices plus `ghost` points on each side; the parent

@@ -79,6 +79,8 @@ _SIGNATURES = {
     "bssn_norm_partial_size": (ctypes.c_int64, [ctypes.POINTER(CGrid)]),
     "bssn_constraint_norms": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p]),
+    "bssn_constraint_norms_noalloc": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_void_p]),
     "bssn_fmr_prolong": (ctypes.c_int, [ctypes.POINTER(CGrid), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                                         ctypes.c_void_p, ctypes.POINTER(CGrid), ctypes.POINTER(ctypes.c_int32),
                                         ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),

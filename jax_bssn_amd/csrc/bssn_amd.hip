@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "bssn_amd.h"
@@ -365,6 +366,13 @@ __global__ void __launch_bounds__(64) k_norm_final(int nblocks, int64_t n, const
   norms[q] = v;
 }
 
+// Device scratch of bssn_constraint_norms_noalloc: the reduction's partial
+// sums (at most RED_BLOCKS_MAX blocks x 10 values, 80 KB).  One copy per
+// device, held until the process ends.  In host code this name is only a
+// stand-in for the symbol; the device address comes from
+// hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) double scratch[RED_BLOCKS_MAX * 10];
+
 }  // namespace bssn
 
 // =================================================================== host
@@ -519,6 +527,38 @@ bool valid_part(int part) {
   return part == PART_TILES || part == PART_SHELL || part == PART_LEAN || part == PART_CLOSURE;
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_address(double **out) {
+  int dev = 0;
+  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(BSSN_ERR_UNSUPPORTED, "device %d beyond the %d the scratch table holds", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(bssn::scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BSSN_OK;
+}
+
+int launch_norms(int64_t n, const double *constraints, double *partial, double *norms, hipStream_t st) {
+  int64_t blocks = (n + RED_THREADS - 1) / RED_THREADS;
+  if (blocks > RED_BLOCKS_MAX) blocks = RED_BLOCKS_MAX;
+  hipLaunchKernelGGL(k_norm_partial, dim3((unsigned)blocks), dim3(RED_THREADS), 0, st, n, constraints, partial);
+  int rc = check_hip(hipGetLastError(), "norm kernel launch");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, st, (int)blocks, n, partial, norms);
+  return check_hip(hipGetLastError(), "norm kernel launch");
+}
+
 int launch_project(int64_t n, const double *in, double *out, hipStream_t st) {
   const int64_t blocks = (n + 255) / 256;
   hipLaunchKernelGGL(k_project, dim3((unsigned)blocks), dim3(256), 0, st, n, in, out);
@@ -670,15 +710,22 @@ int bssn_constraint_norms(const bssn_grid *grid, const double *constraints, doub
                           void *stream) {
   if (!grid || !constraints || !partial || !norms) return fail(BSSN_ERR_INVALID, "null argument");
   const int64_t n = (int64_t)grid->nx * grid->ny * grid->nz;
-  if (n <= 0) return fail(BSSN_ERR_INVALID, "empty grid");
-  int64_t blocks = (n + RED_THREADS - 1) / RED_THREADS;
-  if (blocks > RED_BLOCKS_MAX) blocks = RED_BLOCKS_MAX;
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_norm_partial, dim3((unsigned)blocks), dim3(RED_THREADS), 0, st, n, constraints, partial);
-  int rc = check_hip(hipGetLastError(), "norm kernel launch");
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail(BSSN_ERR_INVALID, "empty grid");
+  return launch_norms(n, constraints, partial, norms, (hipStream_t)stream);
+}
+
+int bssn_constraint_norms_noalloc(const bssn_grid *grid, const double *constraints, double *norms, void *stream) {
+  if (!grid || !constraints || !norms) return fail(BSSN_ERR_INVALID, "null argument");
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail(BSSN_ERR_INVALID, "empty grid");
+  const int64_t n = (int64_t)grid->nx * grid->ny * grid->nz;
+  if (n >= ((int64_t)1 << 31))
+    return fail(BSSN_ERR_INVALID, "the allocation-free norms take at most 2^31 - 1 points per component (got %lld)",
+                (long long)n);
+  double *partial = nullptr;
+  const int rc = scratch_address(&partial);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, st, (int)blocks, n, partial, norms);
-  return check_hip(hipGetLastError(), "norm kernel launch");
+  // k_norm_partial writes every slot k_norm_final reads: nothing to clear
+  return launch_norms(n, constraints, partial, norms, (hipStream_t)stream);
 }
 
 int bssn_time_stage_part(const bssn_params *p, const bssn_grid *grid, int precision, int stage, int part,
