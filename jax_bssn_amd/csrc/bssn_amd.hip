@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(TX *TY *TZ) k_kappa_tile(Ctx C, const double *
                                                           const double *__restrict__ matter, double *kap,
                                                           TileSet S) {
   using P = TileProvider<double, TX, TY, TZ, false, CL>;
-  __shared__ double box[P::NBOX];
+  __shared__ double box[P::NBOXP];
   int bx, by, bz, slab;
   tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz, &slab);
   const int tid = threadIdx.x;
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(RTX *RTY *RTZ) k_kappa_region(Ctx C, const dou
                                                                const double *__restrict__ matter, double *kap,
                                                                RegionTiles R) {
   using P = TileProvider<double, RTX, RTY, RTZ, false, true, 5>;
-  __shared__ double box[P::NBOX];
+  __shared__ double box[P::NBOXP];
   const int tile = (int)blockIdx.x;
   const int x0 = R.lo[0] + (tile / R.nz) * RTX, y0 = R.lo[1], z0 = R.lo[2] + (tile % R.nz) * RTZ;
   const int tid = threadIdx.x;

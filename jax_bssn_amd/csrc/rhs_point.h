@@ -114,6 +114,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
   T db[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
+    out.prefetch(BETA + i);
     pv.stage(i);
     if (face) out.put_face(BETA + i, V(pv.somm(BETA + i, u[BETA + i])));
 #pragma unroll
@@ -148,6 +149,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
   T dg[6][3];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
+    out.prefetch(GXX + c);
     pv.stage(3 + c);
     if (face) out.put_face(GXX + c, V(pv.somm(GXX + c, u[GXX + c])));
 #pragma unroll
@@ -169,6 +171,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
   // ---- lapse (visit 9): d_i alpha and W^2 d_i d_j alpha
   T da[3], Hs[6];
   {
+    out.prefetch(ALPHA);
     pv.stage(9);
     if (face) out.put_face(ALPHA, V(pv.somm(ALPHA, u[ALPHA])));
 #pragma unroll
@@ -184,6 +187,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
   // ---- conformal factor (visit 10): W (d_i d_j W + gamma_ij gamma^mn d_m d_n W)
   T dW[3];
   {
+    out.prefetch(CW);
     pv.stage(10);
     if (face) out.put_face(CW, V(pv.somm(CW, u[CW])));
 #pragma unroll
@@ -198,6 +202,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
 
   // ---- trace K (visit 11): -4/3 alpha gamma^ij d_j K goes straight to the connection
   {
+    out.prefetch(CK);
     pv.stage(11);
     if (face) out.put_face(CK, V(pv.somm(CK, u[CK])));
     T dK[3];
@@ -212,6 +217,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
   // ---- connection (visits 12..14): W^2 gamma_m(i d_j) Gamma^m
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
+    out.prefetch(GAM + m);
     pv.stage(12 + m);
     if (face) out.put_face(GAM + m, V(pv.somm(GAM + m, u[GAM + m])));
     T dG[3];
@@ -285,6 +291,12 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
   for (int c = 0; c < 6; ++c) R[c] -= W * g[c] * conW;
 
   const T Wf = W > T(1.0e-12) ? W : T(1.0e-12);
+
+  // operands of the four updates that follow (fetched again here rather
+  // than held across the Christoffel algebra since their visits)
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out.prefetch(GAM + i);
+  out.prefetch(CK);
 
   // ---- A^ij and the connection term 2 alpha Gamma^i_jk A^jk = 2 alpha gamma^im Gamma_mjk A^jk
 #pragma unroll
@@ -366,6 +378,7 @@ BSSN_HD __forceinline__ void bssn_rhs_point(P &pv, const Ctx &C, const V (&u)[NC
   for (int i = 0; i < 3; ++i) b[i] = (T)pv.st.get(ST_B + i);
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
+    out.prefetch(AXX + c);
     pv.stage(15 + c);
     if (face) out.put_face(AXX + c, V(pv.somm(AXX + c, u[AXX + c])));
     out.put(AXX + c, V(Arest[c] + advection(pv, b) + pv.ko()) + out.extra_A(c));

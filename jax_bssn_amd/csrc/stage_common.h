@@ -64,6 +64,7 @@ struct Emitter {
   bool active, face;
   S st;  // holds the metric of the stage update until done() (its own copy of the handle)
   double An[6];
+  double pn[NCOMP], ps[NCOMP];  // stage-update operands loaded by prefetch()
 
   BSSN_HD __forceinline__ Emitter(const Ctx &C, const StageOut &O, int64_t i, bool act, bool on_face, const S &stash)
       : n(C.ncell), idx(i), wn(O.wn), ws(O.ws), nbase((const BSSN_GLOBAL double *)O.nbase),
@@ -71,6 +72,17 @@ struct Emitter {
         unext((BSSN_GLOBAL double *)O.unext), rhs((BSSN_GLOBAL double *)O.rhs),
         kap((const BSSN_GLOBAL double *)O.kap), mat((const BSSN_GLOBAL double *)O.matter), mode(O.mode),
         reproject(O.reproject), active(act), face(on_face), st(stash) {}
+
+  // Load the stage-update operands of component c at this point (the
+  // u_next and ustar bases) ahead of its RHS, so that their latency
+  // overlaps the component's stencils.  Every put / put_face of c follows
+  // a prefetch(c).
+  BSSN_HD __forceinline__ void prefetch(int c) {
+    if (!active || mode == 0) return;
+    const int64_t at = c * n + idx;
+    pn[c] = nbase[at];
+    ps[c] = sbase ? sbase[at] : 0.0;
+  }
 
   // bulk RHS of component c (ignored on a Sommerfeld face point)
   BSSN_HD __forceinline__ void put(int c, double k) {
@@ -86,8 +98,8 @@ struct Emitter {
       rhs[at] = k;
       return;
     }
-    const double un = nbase[at] + wn * k;
-    if (sbase) ustar[at] = sbase[at] + ws * k;
+    const double un = pn[c] + wn * k;
+    if (sbase) ustar[at] = ps[c] + ws * k;
     if (c <= GZZ) {
       st.put(ST_GN + c, un);
     } else if (c >= AXX && c <= AZZ) {

@@ -19,9 +19,25 @@
 
 #include "bssn_common.h"
 
-// Components whose LDS boxes are held in registers ahead of their visit.
+// Components whose LDS boxes are held in registers ahead of their visit:
+// in the FP64 closure-free tile kernels (one wave per SIMD, registers to
+// spare) and in the others (two waves per SIMD).
+#ifndef BSSN_PREFETCH_DEPTH_SEL
+#define BSSN_PREFETCH_DEPTH_SEL 2
+#endif
 #ifndef BSSN_PREFETCH_DEPTH
 #define BSSN_PREFETCH_DEPTH 1
+#endif
+// LDS z pitch of the boxes of the FP64 closure-free tile kernels (0: the
+// box's own z extent).  A wave reads an 8 x 8 y-z plane of its box: with
+// rows 24 doubles apart, four consecutive rows of eight cover the 64 banks
+// once, where the natural pitch of 14 makes the rows collide in pairs.
+#ifndef BSSN_LDS_PITCH_SEL
+#define BSSN_LDS_PITCH_SEL 24
+#endif
+// The same for the other tile kernels (0: natural pitch).
+#ifndef BSSN_LDS_PITCH
+#define BSSN_LDS_PITCH 0
 #endif
 
 namespace bssn {
@@ -223,6 +239,16 @@ BSSN_HD __forceinline__ Row8<T> row_d6(const Axis &A, int p) {
   return r;
 }
 
+// Workgroup barrier that orders LDS only: every wave's LDS writes are done
+// and visible after it, while its global loads stay in flight (a full
+// __syncthreads() also drains them, so a prefetch issued before it could
+// not overlap the next component's work).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ------------------------------------------------------------------ stash
 
 // Per-thread parking space (slots in bssn_common.h) for values that stay
@@ -273,14 +299,23 @@ struct TileProvider {
   static constexpr int HX = tile_halo(TX, MAD, CLX), HY = tile_halo(TY, MAD, CLY), HZ = tile_halo(TZ, MAD, CLZ);
   static constexpr int NT = TX * TY * TZ;
   static constexpr int LX = TX + 2 * HX, LY = TY + 2 * HY, LZ = TZ + 2 * HZ;
-  static constexpr int SX = LY * LZ, SY = LZ;
-  static constexpr int NBOX = LX * LY * LZ;
+  // z pitch of the LDS box (rows of LZ samples; SEL kernels may pad it so
+  // that the rows a wave reads fall in distinct banks)
+  static constexpr int LZP = SEL ? (BSSN_LDS_PITCH_SEL > LZ ? BSSN_LDS_PITCH_SEL : LZ)
+                                 : (BSSN_LDS_PITCH > LZ ? BSSN_LDS_PITCH : LZ);
+  static constexpr int SX = LY * LZP, SY = LZP;
+  static constexpr int NBOX = LX * LY * LZ;     // samples staged per component
+  static constexpr int NBOXP = LX * LY * LZP;   // LDS elements of one box buffer
   static constexpr int NSLOT = (NBOX + NT - 1) / NT;
-  // Double-buffered when two workgroups (box buffers + stash) still fit the
-  // 160 KB of LDS of a CU; wider boxes (closure or MAD halos) keep one.
+  // Double-buffered when the workgroups a CU holds (box buffers + stash)
+  // still fit its 160 KB of LDS: one workgroup of the SEL kernels (one wave
+  // per SIMD), two of the others; wider boxes (closure or MAD halos) keep
+  // one buffer.
   static constexpr int NBUF =
-      (2 * NBOX * (int)sizeof(T) + STASH_SLOTS * NT * (int)sizeof(double) <= 80 * 1024) ? 2 : 1;
-  static constexpr int LDS_ELEMS = NBUF * NBOX;
+      (2 * NBOXP * (int)sizeof(T) + STASH_SLOTS * NT * (int)sizeof(double) <= (SEL ? 160 : 80) * 1024) ? 2 : 1;
+  static constexpr int LDS_ELEMS = NBUF * NBOXP;
+  // LDS element of staged sample e (row-major over LX x LY rows of LZ)
+  __device__ __forceinline__ static int lds_index(int e) { return LZP == LZ ? e : e + (e / LZ) * (LZP - LZ); }
 
   BSSN_LDS T *lds;        // two tile buffers
   const BSSN_LDS T *cur;  // buffer of the staged component
@@ -288,10 +323,10 @@ struct TileProvider {
   const BSSN_GLOBAL double *src;
   int64_t ncell;
   int goff[NSLOT];
-  int boff;       // tile origin (the FP32 offset of each component)
+  int boff;       // FP32 offset sample of each component (the clamped tile origin)
   // register prefetch of the next PD components' tiles (ring indexed by the
   // visit number, a compile-time constant in the unrolled per-point code)
-  static constexpr int PD = BSSN_PREFETCH_DEPTH;
+  static constexpr int PD = SEL ? BSSN_PREFETCH_DEPTH_SEL : BSSN_PREFETCH_DEPTH;
   T pref[PD][NSLOT];
   int c;
   int tid;
@@ -327,7 +362,7 @@ struct TileProvider {
     ncell = C.ncell;
     tid = threadIdx.x;  // the cooperative loads and the stash follow the lanes, not the point
     st.base = (BSSN_LDS double *)stash_mem + tid;
-    c = ((tx + HX) * LY + (ty + HY)) * LZ + (tz + HZ);
+    c = ((tx + HX) * LY + (ty + HY)) * LZP + (tz + HZ);
     inv_dx = (T)C.inv_dx;
     inv_dx2 = (T)C.inv_dx2;
     ko_coef = (T)C.ko;
@@ -360,13 +395,24 @@ struct TileProvider {
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;
       if (e >= NBOX) e = NBOX - 1;
-      int lx = e / SX, ly = (e / LZ) % LY, lz = e % LZ;
+      int lx = e / (LY * LZ), ly = (e / LZ) % LY, lz = e % LZ;
       int gx = wrap_index(x0 + lx - HX, C.ax[0].n);
       int gy = wrap_index(y0 + ly - HY, C.ax[1].n);
       int gz = wrap_index(z0 + lz - HZ, C.ax[2].n);
       goff[k] = gx * C.ax[0].stride + gy * C.ax[1].stride + gz;
     }
-    boff = x0 * C.ax[0].stride + y0 * C.ax[1].stride + z0;
+    {
+      // the FP32 offset sample: the tile origin clamped into the owned box,
+      // so that it is within the stencils' reach of every owned point the
+      // tile updates (a tile may start in guard samples beyond that reach)
+      int b[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int o = a == 0 ? x0 : (a == 1 ? y0 : z0);
+        b[a] = o < C.ax[a].start ? C.ax[a].start : (o >= C.ax[a].end ? C.ax[a].end - 1 : o);
+      }
+      boff = b[0] * C.ax[0].stride + b[1] * C.ax[1].stride + b[2];
+    }
     if (prefetch) {
 #pragma unroll
       for (int d = 0; d < PD; ++d) fetch(visit_comp(d), d);
@@ -385,7 +431,7 @@ struct TileProvider {
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;
-      if (k < NSLOT - 1 || e < NBOX) lds[e] = pref[0][k];
+      if (k < NSLOT - 1 || e < NBOX) lds[lds_index(e)] = pref[0][k];
     }
     __syncthreads();
     cur = lds;
@@ -397,15 +443,15 @@ struct TileProvider {
   // barrier of visit v - 1; a single buffer needs a second barrier before it
   // is overwritten.
   __device__ __forceinline__ void stage(int v) {
-    BSSN_LDS T *buf = lds + (NBUF == 2 ? (v & 1) * NBOX : 0);
+    BSSN_LDS T *buf = lds + (NBUF == 2 ? (v & 1) * NBOXP : 0);
     const int slot = v % PD;
-    if (NBUF == 1 && v > 0) __syncthreads();  // every thread is done with visit v - 1
+    if (NBUF == 1 && v > 0) lds_barrier();  // every thread is done with visit v - 1
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       int e = tid + k * NT;
-      if (k < NSLOT - 1 || e < NBOX) buf[e] = pref[slot][k];
+      if (k < NSLOT - 1 || e < NBOX) buf[lds_index(e)] = pref[slot][k];
     }
-    __syncthreads();
+    lds_barrier();
     if (v + PD < NCOMP) fetch(visit_comp(v + PD), slot);
     cur = buf;
   }

@@ -38,8 +38,8 @@ template <int ND, int TX_, int TY_, int TZ_, bool CL>
 struct DualTileProvider {
   using B = TileProvider<double, TX_, TY_, TZ_, false, CL>;
   using V = DualN<ND>;
-  static constexpr int NBOX = B::NBOX, NSLOT = B::NSLOT, NT = B::NT;
-  static constexpr int LDS_ELEMS = (ND + 1) * NBOX;
+  static constexpr int NBOX = B::NBOX, NBOXP = B::NBOXP, NSLOT = B::NSLOT, NT = B::NT;
+  static constexpr int LDS_ELEMS = (ND + 1) * NBOXP;
   mutable B b;  // geometry, closure rows and stencils; b.cur selects the box
   StashT<V> st;
   BSSN_LDS double *lds;
@@ -69,11 +69,11 @@ struct DualTileProvider {
     if (v > 0) __syncthreads();
 #pragma unroll
     for (int l = 0; l <= ND; ++l) {
-      BSSN_LDS double *buf = lds + l * NBOX;
+      BSSN_LDS double *buf = lds + l * NBOXP;
 #pragma unroll
       for (int k = 0; k < NSLOT; ++k) {
         const int e = b.tid + k * NT;
-        if (k < NSLOT - 1 || e < NBOX) buf[e] = t[l][k];
+        if (k < NSLOT - 1 || e < NBOX) buf[B::lds_index(e)] = t[l][k];
       }
     }
     __syncthreads();
@@ -85,7 +85,7 @@ struct DualTileProvider {
     r.v = fn();
 #pragma unroll
     for (int l = 0; l < ND; ++l) {
-      b.cur = lds + (l + 1) * NBOX;
+      b.cur = lds + (l + 1) * NBOXP;
       r.d[l] = fn();
     }
     return r;
@@ -152,6 +152,7 @@ struct DualEmitter {
     if (!face) write(c, k);
   }
   __device__ __forceinline__ void put_face(int c, V k) { write(c, k); }
+  __device__ __forceinline__ void prefetch(int) {}
   __device__ __forceinline__ void store(double *v, double *dv, int64_t at, const V &x) {
     if (value_out) v[at] = x.v;
 #pragma unroll
