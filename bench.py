@@ -27,6 +27,19 @@ HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP64_PEAK = 78.6e12        # FLOP/s, MI355X FP64 vector
 BYTES_PER_CELL = {1: 3 * 21 * 8, 2: 5 * 21 * 8, 3: 5 * 21 * 8, 4: 3 * 21 * 8}  # algorithmic, per stage
 
+# Algorithmic FP64 operations (add/sub, mul, div, sqrt, cbrt one each; a
+# fused multiply-add two), counted by tests/hostcheck/flop_count.cpp:
+# the per-point RHS (kappa = 0, vacuum) with the reference's stencil rows as
+# dot products over their non-zero taps, at an interior point; one
+# algebraic projection; the RK4 combinations of the 21 components (two
+# axpys per component in stages 1-3, one in stage 4).  An RK4 step makes
+# four RHS evaluations and five projections (stage inputs 2-4, and the
+# output projected twice where consecutive steps meet).
+FLOPS_RHS = 5344
+FLOPS_PROJECTION = 82
+FLOPS_STAGE_UPDATES = 21 * (4 + 4 + 4 + 2)
+F_ALG = 4 * FLOPS_RHS + 5 * FLOPS_PROJECTION + FLOPS_STAGE_UPDATES  # per cell-update
+
 
 def puncture_state(n, L=8.0, mass=1.0, device="cuda"):
     import torch
@@ -480,6 +493,9 @@ def main():
                                "Sommerfeld faces, KO nu=0.1, dt=dx/4, FP64 RK4",
                    "grid": [n, n, n], "global_batch": 1, "parallelism": "single GPU"},
         "roofline": roof,
+        # the whole step against the FP64 vector peak
+        "fp64_compute": {"flops_per_cell_update": F_ALG, "achieved": rate64 * F_ALG / 1e12,
+                         "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s", "frac": rate64 * F_ALG / FP64_PEAK},
         "closure_tile_kernel": {k: closure[k] for k in ("kernel", "achieved", "frac", "launch_ms_avg",
                                                          "points_per_launch")},
         # gradient of the L2 Hamiltonian constraint w.r.t. two initial-data

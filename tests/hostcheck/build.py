@@ -7,8 +7,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 LIB = os.path.join(HERE, "libhostcheck.so")
-SRC = os.path.join(HERE, "host_rhs.cpp")
-DEPS = [SRC] + [os.path.join(ROOT, "jax_bssn_amd", "csrc", f) for f in
+SRCS = [os.path.join(HERE, "host_rhs.cpp"), os.path.join(HERE, "flop_count.cpp")]
+DEPS = SRCS + [os.path.join(ROOT, "jax_bssn_amd", "csrc", f) for f in
                 ("bssn_common.h", "rhs_point.h", "stage_common.h", "stencil_providers.h", "tiles.h")] + \
        [os.path.join(ROOT, "include", "bssn_amd.h")]
 
@@ -23,7 +23,7 @@ def stale():
 def build():
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     cmd = [os.path.join(rocm, "bin", "hipcc"), "-O2", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
-           f"-I{ROOT}/include", f"-I{ROOT}/jax_bssn_amd/csrc", SRC, "-o", LIB + ".tmp"]
+           f"-I{ROOT}/include", f"-I{ROOT}/jax_bssn_amd/csrc", *SRCS, "-o", LIB + ".tmp"]
     subprocess.run(cmd, check=True, capture_output=True)
     os.replace(LIB + ".tmp", LIB)
     return LIB
