@@ -361,6 +361,56 @@ def tangent_gradient(J, n=128, steps=16):
             "autograd_call_seconds": t_e2e}
 
 
+def adjoint_gradient(J, n=128, steps=16, checkpoint_every=4, forward_mode=None):
+    """The same gradient as ``tangent_gradient`` (L2 Hamiltonian constraint
+    after ``steps`` RK4 steps of the single puncture with respect to M and
+    eps) by the reverse sweep: the forward evolution storing a checkpoint
+    every ``checkpoint_every`` steps, the adjoint of the objective, then the
+    steps backwards (each segment recomputed from its checkpoint), the
+    gradient being the initial adjoint against the initial-data
+    derivatives.  Timed against the plain forward evolution of the same
+    steps and compared with the forward-mode gradient."""
+    import numpy as np
+    import torch
+
+    from jax_bssn_amd.adjoint import _Ops, evolve_vjp, hamiltonian_l2_vjp
+    from jax_bssn_amd.tangent import puncture_initial
+    from jax_bssn_amd.time_evolve import evolve_packed
+
+    L = 8.0
+    dx = 2 * L / n
+    x = -L + (np.arange(n) + 0.5) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    c = np.random.default_rng(0).uniform(-1, 1, 3)
+    initial = puncture_initial(X, Y, Z, width=1.0, lapse_centre=tuple(c))
+    p = puncture_params(J.BSSNParameters, dx, x[0])
+    u0, tangents = initial(np.array([1.0, 1e-3]))
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    ops = _Ops(u0, p)
+    timed(lambda: evolve_packed(u0.clone(), p, 1))
+    t_fwd, _ = timed(lambda: evolve_packed(u0.clone(), p, steps))
+
+    def value_and_gradient():
+        ubar, value = evolve_vjp(u0, steps, p, lambda u: hamiltonian_l2_vjp(u, p, ops), checkpoint_every, ops)
+        return value, [float(torch.sum(ubar * t)) for t in tangents]
+    t_grad, (val, grad) = timed(value_and_gradient)
+    out = {"method": f"reverse-mode adjoint, checkpoint every {checkpoint_every} steps, segments recomputed",
+           "grid": [n, n, n], "steps": steps, "parameters": ["M", "eps"], "hamiltonian_l2": val,
+           "gradient": grad, "forward_seconds": t_fwd, "value_and_gradient_seconds": t_grad,
+           "gradient_forward_ratio": t_grad / t_fwd}
+    if forward_mode:
+        fm = forward_mode["gradient"]
+        out["rel_diff_vs_forward_mode"] = [abs(a - b) / abs(b) for a, b in zip(grad, fm)]
+    return out
+
+
 def copy_calibration(nbytes=1 << 31, reps=3):
     """A plain device copy of known size (read nbytes, write nbytes), for
     scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -436,6 +486,8 @@ def main():
     ap.add_argument("--no-fmr", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=50,
                     help="steps of the FP64 / FP32-RHS comparison run (0: none)")
+    ap.add_argument("--adjoint-size", type=int, default=0,
+                    help="grid edge of the reverse-mode gradient leg (0: --size)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -501,7 +553,7 @@ def main():
         # gradient of the L2 Hamiltonian constraint w.r.t. two initial-data
         # parameters over its time of the forward evolution (filled below)
         "gradient_forward_ratio": None,
-        "gradient_method": "forward-mode tangent, both parameters in one run (no reverse-mode adjoint)",
+        "gradient_method": None,
     }
     if not args.no_batch:
         extra["batched_gauge_waves"] = batched_gauge_waves(J, args.steps)
@@ -519,7 +571,10 @@ def main():
         extra["fmr_linear_wave"] = fmr_linear_wave(J, args.steps)
     if not args.no_gradient:
         extra["tangent_gradient"] = tangent_gradient(J, n=n)
-        line["gradient_forward_ratio"] = extra["tangent_gradient"]["gradient_forward_ratio"]
+        extra["adjoint_gradient"] = adjoint_gradient(J, n=args.adjoint_size or n, forward_mode=(
+            extra["tangent_gradient"] if (args.adjoint_size or n) == n else None))
+        line["gradient_forward_ratio"] = extra["adjoint_gradient"]["gradient_forward_ratio"]
+        line["gradient_method"] = extra["adjoint_gradient"]["method"]
     line.update(extra)
     if args.copy_calibration:
         copy_calibration()

@@ -298,6 +298,27 @@ int bssn_evolve_tangent(const bssn_params *p, const bssn_grid *grid, int nsteps,
 int bssn_hamiltonian_tangent(const bssn_params *p, const bssn_grid *grid, int ndir, const double *state,
                              const double *dstate, double *H, double *dH, void *stream);
 
+/* Reverse-mode differentiation (no reference counterpart: the reference
+ * differentiates with jax autodiff): the transposed linearisations an
+ * adjoint (reverse) sweep of rk4_step (time_evolve.py:144-217) is built
+ * from.  kappa = 0, vacuum, whole grids (no guard cells; status
+ * BSSN_ERR_UNSUPPORTED otherwise).
+ *
+ * bssn_rhs_vjp: state_bar (= or += when accumulate) <- J_F(state)^T rhs_bar
+ * + J_H(state)^T h_bar, with F the RHS of compute_bssn_rhs (time_evolve.py:
+ * 41-57) and H the Hamiltonian constraint (bssn/constraints.py:271-303);
+ * rhs_bar (21 * n) or h_bar (n) may be null.  work holds
+ * bssn_vjp_work_size(grid) doubles.  The upwind side of each point is held
+ * fixed, as jax differentiates jnp.where. */
+int64_t bssn_vjp_work_size(const bssn_grid *grid);
+int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *state, const double *rhs_bar,
+                 const double *h_bar, double *state_bar, int accumulate, double *work, void *stream);
+
+/* state_bar <- J_P(state)^T state_bar in place, P the pointwise projection
+ * of enforce_algebraic_constraints (time_evolve.py:94-141) at the
+ * pre-projection state. */
+int bssn_project_vjp(const bssn_grid *grid, const double *state, double *state_bar, void *stream);
+
 /* Launch parts of one RK4 stage, for the measurement helpers below:
  * TILES    both tile kernels (every point the tile grid owns);
  * SHELL    the Sommerfeld closure-shell kernel (tiles not aligned to faces);

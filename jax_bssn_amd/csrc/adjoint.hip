@@ -1,0 +1,224 @@
+// Reverse-mode differentiation (the adjoint) of the RHS, of the Hamiltonian
+// constraint and of the algebraic projection: the transposed linearisations
+// the reverse sweep of an RK4 evolution is built from (jax_bssn_amd/
+// adjoint.py composes them, with checkpoints, into the gradient of the
+// evolution).
+//
+// bssn_rhs_vjp forms state_bar = J_F(state)^T rhs_bar + J_H(state)^T h_bar
+// in two launches:
+//  * k_vjp_point: at every point, the features of the point (values, first
+//    and second derivatives, upwind derivatives and Kreiss-Oliger sums of
+//    every component, with the reference's stencil rows: closures, MAD,
+//    upwind side by the sign of the shift at the point) and the adjoints of
+//    those features (adjoint_point.h), written as NFEAT grid functions;
+//  * k_vjp_transpose: at every point q, each component's adjoint is the
+//    sum, over every point p whose stencils read q, of the row coefficient
+//    of p at the offset q - p times p's feature adjoint -- the transposed
+//    stencils, rows evaluated at p (a one-sided closure row, or the upwind
+//    side chosen by p's own shift).
+// kappa = 0, vacuum, whole grids (no guard cells) only.
+#include <hip/hip_runtime.h>
+
+#include "adjoint_point.h"
+#include "bssn_amd.h"
+#include "host_errors.h"
+#include "stage_common.h"
+
+namespace bssn {
+namespace {
+
+__device__ __forceinline__ void point_of(const Ctx &C, int64_t i, int (&q)[3]) {
+  q[2] = (int)(i % C.ax[2].n);
+  q[1] = (int)((i / C.ax[2].n) % C.ax[1].n);
+  q[0] = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
+}
+
+__global__ void __launch_bounds__(64) k_vjp_point(Ctx C, const double *__restrict__ us,
+                                                  const double *__restrict__ rbar, const double *__restrict__ hbar,
+                                                  double *__restrict__ fbar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.ncell) return;
+  const int64_t n = C.ncell;
+  int q[3];
+  point_of(C, i, q);
+  PointProvider<double> pv;
+  pv.setup(C, us, q[0], q[1], q[2]);
+  const bool face = rbar && pv.face();
+  double f[NFEAT];
+  double b[3];
+  for (int a = 0; a < 3; ++a) b[a] = us[(BETA + a) * n + i];
+  for (int c = 0; c < NCOMP; ++c) {
+    pv.stage_comp(c);
+    f[F_VAL + c] = us[c * n + i];
+    const bool with_d1 = c < AXX || c > AZZ || face;
+    for (int a = 0; a < 3; ++a) {
+      f[F_D1 + 3 * c + a] = with_d1 ? pv.d1(a) : 0.0;
+      f[F_UP + 3 * c + a] = pv.upw(a, b[a]);
+    }
+    f[F_KO + c] = pv.ko();
+    const int s = d2_slot(c);
+    if (s >= 0) {
+      double dd[6];
+      second_derivatives(pv, dd);
+      for (int p = 0; p < 6; ++p) f[F_D2 + 6 * s + p] = dd[p];
+    }
+  }
+  double fb[NFEAT];
+  for (int k = 0; k < NFEAT; ++k) fb[k] = 0.0;
+  double rb[NCOMP];
+  if (rbar)
+    for (int c = 0; c < NCOMP; ++c) rb[c] = rbar[c * n + i];
+  const double hb = hbar ? hbar[i] : 0.0;
+  if (face) {
+    // the Sommerfeld RHS replaces the bulk one on an active face point
+    double xr[3];
+    for (int a = 0; a < 3; ++a) xr[a] = C.xmin[a] + C.dx * (double)q[a];
+    face_vjp_point(rb, xr, fb);
+    rhs_vjp_point(C, f, (const double *)nullptr, hb, fb);
+  } else {
+    rhs_vjp_point(C, f, rbar ? rb : (const double *)nullptr, hb, fb);
+  }
+  for (int k = 0; k < NFEAT; ++k) fbar[k * n + i] = fb[k];
+}
+
+// coefficient of the tap at offset d of a row (0 outside its reach)
+__device__ __forceinline__ double tap(const Row8<double> &r, int d) {
+  const int k = d - r.o0;
+  return (k >= 0 && k < r.n) ? r.c[k] : 0.0;
+}
+
+// offsets of the points whose rows along axis a may read q: the one-sided
+// rows reach seven samples inward from a face, the centred ones three (four
+// for the MAD upwind rows)
+__device__ __forceinline__ int reach(const Axis &A, int qa) {
+  const bool near = (A.lbc && qa - A.start < 10) || (A.rbc && A.end - 1 - qa < 10);
+  const int r = near ? 7 : (A.q != 1.0 ? 4 : 3);
+  return r < A.n ? r : A.n - 1;
+}
+
+__global__ void __launch_bounds__(128) k_vjp_transpose(Ctx C, const double *__restrict__ us,
+                                                       const double *__restrict__ fbar, double *__restrict__ ubar,
+                                                       int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.ncell) return;
+  const int64_t n = C.ncell;
+  int q[3];
+  point_of(C, i, q);
+  double acc[NCOMP];
+  for (int c = 0; c < NCOMP; ++c) acc[c] = fbar[(F_VAL + c) * n + i];
+  // first, pure second, upwind and Kreiss-Oliger rows along each axis
+  for (int a = 0; a < 3; ++a) {
+    const Axis &A = C.ax[a];
+    if (A.reduced) continue;
+    const int R = reach(A, q[a]);
+    for (int d = -R; d <= R; ++d) {
+      int p[3] = {q[0], q[1], q[2]};
+      p[a] = wrap_index(q[a] - d, A.n);
+      const int64_t j = (int64_t)p[0] * C.ax[0].stride + (int64_t)p[1] * C.ax[1].stride + p[2];
+      const double c1 = tap(row_d1<double>(A, p[a], C.inv_dx, false), d);
+      const double c2 = tap(row_d2<double>(A, p[a], C.inv_dx2), d);
+      const double c6 = C.ko * tap(row_d6<double>(A, p[a]), d);
+      const double cu = tap(row_upwind<double>(A, p[a], C.inv_dx, us[(BETA + a) * n + j]), d);
+      if (c1 == 0.0 && c2 == 0.0 && c6 == 0.0 && cu == 0.0) continue;
+      for (int c = 0; c < NCOMP; ++c) {
+        double v = c1 * fbar[(F_D1 + 3 * c + a) * n + j] + cu * fbar[(F_UP + 3 * c + a) * n + j] +
+                   c6 * fbar[(F_KO + c) * n + j];
+        const int s = d2_slot(c);
+        if (s >= 0) v += c2 * fbar[(F_D2 + 6 * s + sym(a, a)) * n + j];
+        acc[c] += v;
+      }
+    }
+  }
+  // mixed second derivatives: products of the two axes' first-derivative rows
+  for (int a = 0; a < 3; ++a)
+    for (int bx = a + 1; bx < 3; ++bx) {
+      const Axis &A = C.ax[a], &B = C.ax[bx];
+      if (A.reduced || B.reduced) continue;
+      const int Ra = reach(A, q[a]), Rb = reach(B, q[bx]);
+      for (int da = -Ra; da <= Ra; ++da) {
+        const int pa = wrap_index(q[a] - da, A.n);
+        const double ca = tap(row_d1<double>(A, pa, C.inv_dx, false), da);
+        if (ca == 0.0) continue;
+        for (int db = -Rb; db <= Rb; ++db) {
+          const int pb = wrap_index(q[bx] - db, B.n);
+          const double cb = tap(row_d1<double>(B, pb, C.inv_dx, false), db);
+          if (cb == 0.0) continue;
+          int p[3] = {q[0], q[1], q[2]};
+          p[a] = pa;
+          p[bx] = pb;
+          const int64_t j = (int64_t)p[0] * C.ax[0].stride + (int64_t)p[1] * C.ax[1].stride + p[2];
+          for (int c = 0; c < NCOMP; ++c) {
+            const int s = d2_slot(c);
+            if (s >= 0) acc[c] += ca * cb * fbar[(F_D2 + 6 * s + sym(a, bx)) * n + j];
+          }
+        }
+      }
+    }
+  for (int c = 0; c < NCOMP; ++c) ubar[c * n + i] = (accumulate ? ubar[c * n + i] : 0.0) + acc[c];
+}
+
+__global__ void __launch_bounds__(256) k_project_vjp(int64_t n, const double *__restrict__ us,
+                                                     double *__restrict__ ubar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double g[6], A[6], gb[6], Ab[6];
+  for (int c = 0; c < 6; ++c) {
+    g[c] = us[(GXX + c) * n + i];
+    A[c] = us[(AXX + c) * n + i];
+    gb[c] = ubar[(GXX + c) * n + i];
+    Ab[c] = ubar[(AXX + c) * n + i];
+  }
+  project_vjp(g, A, gb, Ab);
+  for (int c = 0; c < 6; ++c) {
+    ubar[(GXX + c) * n + i] = gb[c];
+    ubar[(AXX + c) * n + i] = Ab[c];
+  }
+}
+
+int resolve_vjp(const bssn_params *p, const bssn_grid *grid, Ctx *C) {
+  char err[512] = {0};
+  const int rc = resolve(p, grid, C, err, sizeof err);
+  if (rc) return fail(rc, "%s", err);
+  if (C->kappa != 0.0) return fail(BSSN_ERR_UNSUPPORTED, "the adjoint takes kappa = 0");
+  if (p->guard_cells != 0 || p->owned_shape[0] >= 0)
+    return fail(BSSN_ERR_UNSUPPORTED, "the adjoint takes whole grids (no guard cells or owned range)");
+  return BSSN_OK;
+}
+
+}  // namespace
+}  // namespace bssn
+
+using namespace bssn;
+
+extern "C" {
+
+int64_t bssn_vjp_work_size(const bssn_grid *grid) {
+  if (!grid || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail(BSSN_ERR_INVALID, "invalid grid");
+  return (int64_t)NFEAT * grid->nx * grid->ny * grid->nz;
+}
+
+int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *state, const double *rhs_bar,
+                 const double *h_bar, double *state_bar, int accumulate, double *work, void *stream) {
+  Ctx C;
+  int rc = resolve_vjp(p, grid, &C);
+  if (rc) return rc;
+  if (!state || !state_bar || !work) return fail(BSSN_ERR_INVALID, "null buffer");
+  if (!rhs_bar && !h_bar) return fail(BSSN_ERR_INVALID, "rhs_bar and h_bar are both null");
+  const hipStream_t st = (hipStream_t)stream;
+  const int64_t b64 = (C.ncell + 63) / 64, b128 = (C.ncell + 127) / 128;
+  hipLaunchKernelGGL(k_vjp_point, dim3((unsigned)b64), dim3(64), 0, st, C, state, rhs_bar, h_bar, work);
+  if ((rc = check_hip(hipGetLastError(), "adjoint point kernel launch"))) return rc;
+  hipLaunchKernelGGL(k_vjp_transpose, dim3((unsigned)b128), dim3(128), 0, st, C, state, work, state_bar, accumulate);
+  return check_hip(hipGetLastError(), "adjoint transpose kernel launch");
+}
+
+int bssn_project_vjp(const bssn_grid *grid, const double *state, double *state_bar, void *stream) {
+  if (!grid || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail(BSSN_ERR_INVALID, "invalid grid");
+  if (!state || !state_bar) return fail(BSSN_ERR_INVALID, "null buffer");
+  const int64_t n = (int64_t)grid->nx * grid->ny * grid->nz;
+  hipLaunchKernelGGL(k_project_vjp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, state,
+                     state_bar);
+  return check_hip(hipGetLastError(), "adjoint projection kernel launch");
+}
+
+}  // extern "C"

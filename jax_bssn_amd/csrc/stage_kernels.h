@@ -22,9 +22,12 @@
 // wave per SIMD (about 500, no scratch); at two they spill 600-1000 B per
 // lane and run 12-40 % slower (256^3 puncture, MI355X).  The closure
 // kernels and the FP32 ones run best at two.
+#ifndef BSSN_F32_INTERIOR_MIN_WAVES
+#define BSSN_F32_INTERIOR_MIN_WAVES 2
+#endif
 template <typename T, bool CL>
 constexpr int tile_min_waves() {
-  return (sizeof(T) == sizeof(double) && !CL) ? 1 : 2;
+  return CL ? 2 : (sizeof(T) == sizeof(double) ? 1 : BSSN_F32_INTERIOR_MIN_WAVES);
 }
 
 namespace bssn {

@@ -7,9 +7,9 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 LIB = os.path.join(HERE, "libhostcheck.so")
-SRCS = [os.path.join(HERE, "host_rhs.cpp"), os.path.join(HERE, "flop_count.cpp")]
+SRCS = [os.path.join(HERE, "host_rhs.cpp"), os.path.join(HERE, "flop_count.cpp"), os.path.join(HERE, "adjoint_check.cpp")]
 DEPS = SRCS + [os.path.join(ROOT, "jax_bssn_amd", "csrc", f) for f in
-                ("bssn_common.h", "rhs_point.h", "stage_common.h", "stencil_providers.h", "tiles.h")] + \
+                ("bssn_common.h", "rhs_point.h", "stage_common.h", "stencil_providers.h", "tiles.h", "adjoint_point.h")] + \
        [os.path.join(ROOT, "include", "bssn_amd.h")]
 
 

@@ -317,6 +317,143 @@ def case_poisoned_guards(L):
         print(f"poisoned guards (precision {precision}): owned RHS unchanged and finite", flush=True)
 
 
+def _vjp(L, s, p, rbar=None, hbar=None):
+    shape = s.shape[1:]
+    cp, cg = _lib.c_params(p), _lib.c_grid(shape)
+    out = np.zeros_like(s)
+    work = np.zeros(L.bssn_vjp_work_size(ctypes.byref(cg)))
+    check(L, L.bssn_rhs_vjp(ctypes.byref(cp), ctypes.byref(cg), ptr(s), None if rbar is None else ptr(rbar),
+                            None if hbar is None else ptr(hbar), ptr(out), 0, ptr(work), None))
+    return out
+
+
+def _dot_test(name, lhs, rhs_, tol=1e-11):
+    err = abs(lhs - rhs_) / max(abs(lhs), abs(rhs_))
+    print(f"{name}: <w, J v> = {lhs:.15e}, <J^T w, v> = {rhs_:.15e}, rel diff {err:.1e} (bar {tol:.0e})", flush=True)
+    if not err <= tol:
+        raise AssertionError(f"{name}: {err}")
+
+
+def _adjoint_configs():
+    return {
+        "periodic 1+log": (smooth_state((12, 14, 16), seed=61),
+                           OB.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=1.0 / 12, dt=0.02, zero_shift=0, gauge=1)),
+        "periodic harmonic, zero shift": (smooth_state((12, 12, 12), seed=62),
+                                          OB.BSSNParameters(eta=0.0, g=0.0, nu=0.1, dx=1.0 / 12, dt=0.02,
+                                                            zero_shift=1, gauge=0)),
+        "Sommerfeld all faces": (smooth_state((16, 17, 18), seed=63), sommerfeld_params()),
+        "Sommerfeld and periodic": (smooth_state((18, 16, 14), seed=64), sommerfeld_params(yl_bc=0, yr_bc=0)),
+        "MAD": (smooth_state((14, 14, 14), seed=65),
+                OB.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=1.0 / 14, dt=0.02, zero_shift=0, gauge=1,
+                                  mad_q=0.25)),
+    }
+
+
+def case_adjoint_rhs(L):
+    """<w, J_F v> from the tangent kernels against <J_F^T w, v> from the
+    adjoint kernels, for the RHS and for the Hamiltonian constraint."""
+    rng = np.random.default_rng(7)
+    for name, (v, p) in _adjoint_configs().items():
+        s = pack(v)
+        shape = s.shape[1:]
+        cp, cg = _lib.c_params(p), _lib.c_grid(shape)
+        dv = np.ascontiguousarray(1e-2 * rng.standard_normal(s.shape))
+        w = np.ascontiguousarray(rng.standard_normal(s.shape))
+        out, dout = np.zeros_like(s), np.zeros_like(s)
+        check(L, L.bssn_rhs_tangent(ctypes.byref(cp), ctypes.byref(cg), 1, ptr(s), ptr(dv), ptr(out), ptr(dout), None))
+        _dot_test(f"RHS, {name}", float(np.sum(w * dout)), float(np.sum(_vjp(L, s, p, rbar=w) * dv)))
+        hb = np.ascontiguousarray(rng.standard_normal(shape))
+        H, dH = np.zeros(shape), np.zeros(shape)
+        check(L, L.bssn_hamiltonian_tangent(ctypes.byref(cp), ctypes.byref(cg), 1, ptr(s), ptr(dv), ptr(H), ptr(dH),
+                                            None))
+        _dot_test(f"H, {name}", float(np.sum(hb * dH)), float(np.sum(_vjp(L, s, p, hbar=hb) * dv)))
+
+
+def case_adjoint_projection(L):
+    rng = np.random.default_rng(8)
+    v = smooth_state((10, 11, 12), seed=2)
+    s = pack(v._replace(conformal_metric=v.conformal_metric * 1.1,
+                        traceless_K=v.traceless_K + 0.05 * v.conformal_metric))
+    cg = _lib.c_grid(s.shape[1:])
+    dv = rng.standard_normal(s.shape)
+    w = np.ascontiguousarray(rng.standard_normal(s.shape))
+
+    def P(x):
+        x = np.ascontiguousarray(x)
+        check(L, L.bssn_enforce_constraints(ctypes.byref(cg), ptr(x), ptr(x), None))
+        return x
+    h = 1e-6
+    lhs = float(np.sum(w * (P(s + h * dv) - P(s - h * dv)) / (2 * h)))
+    wb = w.copy()
+    check(L, L.bssn_project_vjp(ctypes.byref(cg), ptr(s), ptr(wb), None))
+    _dot_test("projection (central differences)", lhs, float(np.sum(wb * dv)), tol=1e-8)
+
+
+def case_adjoint_step(L):
+    """The reverse sweep of one RK4 step composed as jax_bssn_amd/adjoint.py
+    composes it (recomputed stages, transposed RHS and projections, in
+    reverse order) against the tangent of the fused evolution step:
+    <w, J_step v> = <J_step^T w, v>, Sommerfeld and periodic grids."""
+    rng = np.random.default_rng(9)
+    for name, (v, p) in _adjoint_configs().items():
+        if p.mad_q != 1.0:
+            continue
+        s = pack(v)
+        shape = s.shape[1:]
+        cp, cg = _lib.c_params(p), _lib.c_grid(shape)
+        dt = p.dt
+
+        def F(u):
+            return rhs(L, unpack(u), p)
+
+        def P(u):
+            u = np.ascontiguousarray(u.copy())
+            check(L, L.bssn_enforce_constraints(ctypes.byref(cg), ptr(u), ptr(u), None))
+            return u
+
+        def FT(u, kb):
+            return _vjp(L, np.ascontiguousarray(u), p, rbar=np.ascontiguousarray(kb))
+
+        def PT(u, ub):
+            ub = np.ascontiguousarray(ub.copy())
+            check(L, L.bssn_project_vjp(ctypes.byref(cg), ptr(np.ascontiguousarray(u)), ptr(ub), None))
+            return ub
+
+        dv = np.ascontiguousarray(1e-2 * rng.standard_normal(s.shape))
+        w = rng.standard_normal(s.shape)
+        u, du = s.copy(), dv.copy()
+        work = np.zeros(L.bssn_tangent_work_size(ctypes.byref(cg), 1))
+        check(L, L.bssn_evolve_tangent(ctypes.byref(cp), ctypes.byref(cg), 1, 1, ptr(u), ptr(du), ptr(work), None))
+        lhs = float(np.sum(w * du))
+        u0 = P(s)
+        k1 = F(u0)
+        pre1 = u0 + 0.5 * dt * k1
+        m1 = P(pre1)
+        k2 = F(m1)
+        pre2 = u0 + 0.5 * dt * k2
+        m2 = P(pre2)
+        k3 = F(m2)
+        pre3 = u0 + dt * k3
+        m3 = P(pre3)
+        k4 = F(m3)
+        pre_out = u0 + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        pb = PT(pre_out, w)
+        u0b = pb.copy()
+        k1b, k2b, k3b, k4b = dt / 6 * pb, dt / 3 * pb, dt / 3 * pb, dt / 6 * pb
+        b = PT(pre3, FT(m3, k4b))
+        u0b += b
+        k3b += dt * b
+        b = PT(pre2, FT(m2, k3b))
+        u0b += b
+        k2b += 0.5 * dt * b
+        b = PT(pre1, FT(m1, k2b))
+        u0b += b
+        k1b += 0.5 * dt * b
+        u0b += FT(u0, k1b)
+        ub = PT(s, u0b)
+        _dot_test(f"RK4 step, {name}", lhs, float(np.sum(ub * dv)), tol=1e-10)
+
+
 CASES = {name[5:]: fn for name, fn in globals().items() if name.startswith("case_")}
 
 

@@ -1,0 +1,140 @@
+"""GPU tests of the reverse-mode adjoint (jax_bssn_amd/adjoint.py over
+bssn_rhs_vjp / bssn_project_vjp), through the C ABI.
+
+The adjoint of a linear map is pinned by the dot-product identity
+<w, J v> = <J^T w, v> for random v, w, with J v from the forward-mode
+tangent kernels (an independent implementation: dual numbers through the
+evolution's own stage kernels).  Bars: 1e-10 relative for one RHS, H or
+step (both sides are exact derivatives; only rounding separates them), the
+same over several checkpointed steps, and 1e-8 for the gradient of the L2
+Hamiltonian constraint against the forward-mode gradient (the task's
+bar against jax.grad)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import bssn as OB
+from tests.parity_utils import smooth_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def J():
+    import jax_bssn_amd as J
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return J
+
+
+def _dev(J, v):
+    return J.pack(v).to("cuda").contiguous()
+
+
+def _sommerfeld(shape, seed):
+    v = smooth_state(shape, seed=seed)
+    dx = 0.1
+    p = OB.BSSNParameters(eta=2.0, g=0.75, nu=0.1, dx=dx, dt=0.25 * dx, zero_shift=0, gauge=1,
+                          xl_bc=1, xr_bc=1, yl_bc=1, yr_bc=1, zl_bc=1, zr_bc=1, x_min=-0.75, y_min=-0.8,
+                          z_min=-0.83)
+    return v, p
+
+
+def _periodic(shape, seed):
+    v = smooth_state(shape, seed=seed)
+    p = OB.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=1.0 / shape[0], dt=0.2 / shape[0], zero_shift=0, gauge=1)
+    return v, p
+
+
+CASES = {
+    # 24^3 tile-aligned: the tangent side runs the tile kernels with face tiles
+    "sommerfeld_tiles": lambda: _sommerfeld((24, 24, 24), 71),
+    "sommerfeld_points": lambda: _sommerfeld((14, 16, 18), 72),
+    "periodic": lambda: _periodic((16, 16, 16), 73),
+}
+
+
+def _params(J, p):
+    return J.BSSNParameters(**p._asdict())
+
+
+def _check(name, lhs, rhs, tol):
+    err = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"{name}: <w, J v> = {lhs:.15e}  <J^T w, v> = {rhs:.15e}  rel {err:.1e}", flush=True)
+    assert err <= tol, (name, err)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_rhs_and_hamiltonian_adjoints_match_tangents(J, case):
+    from jax_bssn_amd import adjoint as A
+    from jax_bssn_amd import tangent as T
+    v, p = CASES[case]()
+    p = _params(J, p)
+    print(f"[adjoint rhs] case={case}", flush=True)
+    s = _dev(J, v)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    dv = 1e-2 * torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=g)
+    ops = A._Ops(s, p)
+    _, drhs = T.rhs_tangent(s, dv, p)
+    _check(f"RHS {case}", float(torch.sum(w * drhs)), float(torch.sum(ops.FT(s, w) * dv)), 1e-10)
+    hb = torch.randn(s.shape[1:], dtype=torch.float64, device="cuda", generator=g)
+    H = torch.empty(s.shape[1:], dtype=torch.float64, device="cuda")
+    dH = torch.empty_like(H)
+    import ctypes
+    from jax_bssn_amd import _lib
+    from jax_bssn_amd.time_evolve import _ptr, stream_handle
+    _lib.check(_lib.load().bssn_hamiltonian_tangent(ctypes.byref(_lib.c_params(p)),
+                                                    ctypes.byref(_lib.c_grid(s.shape[1:])), 1, _ptr(s), _ptr(dv),
+                                                    _ptr(H), _ptr(dH), stream_handle()))
+    _check(f"H {case}", float(torch.sum(hb * dH)), float(torch.sum(ops.HT(s, hb.contiguous()) * dv)), 1e-10)
+
+
+@pytest.mark.parametrize("case", ["sommerfeld_tiles", "periodic"])
+def test_checkpointed_steps_adjoint_matches_tangent(J, case):
+    """Six steps with a checkpoint every four (one full and one partial
+    segment) against the tangent evolution of the same six steps."""
+    from jax_bssn_amd import adjoint as A
+    from jax_bssn_amd import tangent as T
+    v, p = CASES[case]()
+    p = _params(J, p)
+    print(f"[adjoint steps] case={case}", flush=True)
+    s = _dev(J, v)
+    g = torch.Generator(device="cuda").manual_seed(2)
+    dv = 1e-2 * torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=g)
+    _, du = T.evolve_tangent(s, dv, p, 6)
+    ub, _ = A.evolve_vjp(s, 6, p, w, checkpoint_every=4)
+    _check(f"6 steps {case}", float(torch.sum(w * du)), float(torch.sum(ub * dv)), 1e-10)
+    # one step without checkpoints equals the single-step adjoint
+    _, du1 = T.evolve_tangent(s, dv, p, 1)
+    _check(f"1 step {case}", float(torch.sum(w * du1)), float(torch.sum(A.rk4_step_vjp(s, w, p) * dv)), 1e-10)
+
+
+def test_reverse_gradient_matches_forward_mode_on_a_puncture(J):
+    """Gradient of the L2 Hamiltonian constraint after 8 steps of a single
+    puncture (Sommerfeld faces, 1+log, Gamma driver) with respect to the
+    mass M and a Gaussian lapse perturbation eps: the reverse sweep
+    (checkpoints every 4 steps) against the forward-mode tangents, <= 1e-8."""
+    from jax_bssn_amd.adjoint import HamiltonianL2Reverse
+    from jax_bssn_amd.tangent import HamiltonianL2AfterSteps, puncture_initial
+    import bench
+    n = 24
+    L = 8.0
+    dx = 2 * L / n
+    x = -L + (np.arange(n) + 0.5) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    initial = puncture_initial(X, Y, Z, width=2.0, lapse_centre=(0.5, -0.3, 0.2))
+    p = bench.puncture_params(J.BSSNParameters, dx, x[0])
+    theta = torch.tensor([1.0, 1e-3], dtype=torch.float64, requires_grad=True)
+    rev = HamiltonianL2Reverse.apply(theta, initial, p, 8, 4)
+    rev.backward()
+    g_rev = theta.grad.clone()
+    theta2 = torch.tensor([1.0, 1e-3], dtype=torch.float64, requires_grad=True)
+    fwd = HamiltonianL2AfterSteps.apply(theta2, initial, p, 8)
+    fwd.backward()
+    g_fwd = theta2.grad.clone()
+    print("values", float(rev), float(fwd), "gradients", g_rev.tolist(), g_fwd.tolist(), flush=True)
+    assert abs(float(rev) - float(fwd)) <= 1e-12 * abs(float(fwd))
+    rel = (g_rev - g_fwd).abs() / g_fwd.abs()
+    assert float(rel.max()) <= 1e-8, rel
