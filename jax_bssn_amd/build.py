@@ -17,7 +17,7 @@ LIB_NAME = "libbssn_amd.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = os.environ.get("BSSN_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bssn_amd.hip", "stage_f64.hip", "stage_f32.hip", "transfer.hip", "cartoon.hip", "halo.hip", "batch.hip", "tangent.hip", "tangent_tile1.hip", "tangent_tile2.hip", "adjoint.hip"]
+SOURCES = ["bssn_amd.hip", "stage_f64.hip", "stage_f32.hip", "transfer.hip", "cartoon.hip", "halo.hip", "batch.hip", "tangent.hip", "tangent_tile1.hip", "tangent_tile2.hip", "adjoint.hip", "record_f64.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

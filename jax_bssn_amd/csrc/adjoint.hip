@@ -5,12 +5,13 @@
 // evolution).
 //
 // bssn_rhs_vjp forms state_bar = J_F(state)^T rhs_bar + J_H(state)^T h_bar
-// in two launches:
-//  * k_vjp_point: at every point, the features of the point (values, first
-//    and second derivatives, upwind derivatives and Kreiss-Oliger sums of
-//    every component, with the reference's stencil rows: closures, MAD,
-//    upwind side by the sign of the shift at the point) and the adjoints of
-//    those features (adjoint_point.h), written as NFEAT grid functions;
+// in three steps:
+//  * the stage kernels in recording mode (record_f64.hip): the features of
+//    every point (values, first and second derivatives, upwind derivatives
+//    and Kreiss-Oliger sums of every component, with the reference's
+//    stencil rows: closures, MAD, upwind side by the sign of the shift at
+//    the point) as NFEAT grid functions;
+//  * k_vjp_point: the adjoints of those features (adjoint_point.h);
 //  * k_vjp_transpose: at every point q, each component's adjoint is the
 //    sum, over every point p whose stencils read q, of the row coefficient
 //    of p at the offset q - p times p's feature adjoint -- the transposed
@@ -23,6 +24,7 @@
 #include "bssn_amd.h"
 #include "host_errors.h"
 #include "stage_common.h"
+#include "stage_kernels.h"
 
 namespace bssn {
 namespace {
@@ -33,7 +35,9 @@ __device__ __forceinline__ void point_of(const Ctx &C, int64_t i, int (&q)[3]) {
   q[0] = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
 }
 
-__global__ void __launch_bounds__(64) k_vjp_point(Ctx C, const double *__restrict__ us,
+// The feature adjoints of every point from its recorded features (feat,
+// NFEAT grid functions written by the stage kernels in recording mode).
+__global__ void __launch_bounds__(64) k_vjp_point(Ctx C, const double *__restrict__ feat,
                                                   const double *__restrict__ rbar, const double *__restrict__ hbar,
                                                   double *__restrict__ fbar) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -41,28 +45,11 @@ __global__ void __launch_bounds__(64) k_vjp_point(Ctx C, const double *__restric
   const int64_t n = C.ncell;
   int q[3];
   point_of(C, i, q);
-  PointProvider<double> pv;
-  pv.setup(C, us, q[0], q[1], q[2]);
-  const bool face = rbar && pv.face();
+  const bool face = rbar && sommerfeld_point(C, q[0], q[1], q[2]);
   double f[NFEAT];
-  double b[3];
-  for (int a = 0; a < 3; ++a) b[a] = us[(BETA + a) * n + i];
-  for (int c = 0; c < NCOMP; ++c) {
-    pv.stage_comp(c);
-    f[F_VAL + c] = us[c * n + i];
-    const bool with_d1 = c < AXX || c > AZZ || face;
-    for (int a = 0; a < 3; ++a) {
-      f[F_D1 + 3 * c + a] = with_d1 ? pv.d1(a) : 0.0;
-      f[F_UP + 3 * c + a] = pv.upw(a, b[a]);
-    }
-    f[F_KO + c] = pv.ko();
-    const int s = d2_slot(c);
-    if (s >= 0) {
-      double dd[6];
-      second_derivatives(pv, dd);
-      for (int p = 0; p < 6; ++p) f[F_D2 + 6 * s + p] = dd[p];
-    }
-  }
+  for (int k = 0; k < NFEAT; ++k) f[k] = feat[k * n + i];
+  if (!face)  // A's first derivatives are features of face points only
+    for (int k = F_D1 + 3 * AXX; k < F_D1 + 3 * (AZZ + 1); ++k) f[k] = 0.0;
   double fb[NFEAT];
   for (int k = 0; k < NFEAT; ++k) fb[k] = 0.0;
   double rb[NCOMP];
@@ -86,6 +73,16 @@ __device__ __forceinline__ double tap(const Row8<double> &r, int d) {
   const int k = d - r.o0;
   return (k >= 0 && k < r.n) ? r.c[k] : 0.0;
 }
+
+// The centred rows of points away from closures and MAD blends, as taps
+// at offsets -3 .. 3 (without the 1/dx, 1/dx^2 scales): D4 first and
+// second derivatives, the sixth difference of the dissipation and the
+// forward / backward upwind D4 rows (reference evolution/derivatives.py).
+__device__ constexpr double D1_TAPS[7] = {0.0, 1.0 / 12.0, -2.0 / 3.0, 0.0, 2.0 / 3.0, -1.0 / 12.0, 0.0};
+__device__ constexpr double D2_TAPS[7] = {0.0, -1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0, 0.0};
+__device__ constexpr double D6_TAPS[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+__device__ constexpr double UPF_TAPS[7] = {0.0, 0.0, -3.0 / 12.0, -10.0 / 12.0, 18.0 / 12.0, -6.0 / 12.0, 1.0 / 12.0};
+__device__ constexpr double UPB_TAPS[7] = {-1.0 / 12.0, 6.0 / 12.0, -18.0 / 12.0, 10.0 / 12.0, 3.0 / 12.0, 0.0, 0.0};
 
 // offsets of the points whose rows along axis a may read q: the one-sided
 // rows reach seven samples inward from a face, the centred ones three (four
@@ -111,14 +108,26 @@ __global__ void __launch_bounds__(128) k_vjp_transpose(Ctx C, const double *__re
     const Axis &A = C.ax[a];
     if (A.reduced) continue;
     const int R = reach(A, q[a]);
+    // no closure row and no MAD blend within reach: the centred rows
+    const bool plain = R == 3 && A.q == 1.0;
     for (int d = -R; d <= R; ++d) {
       int p[3] = {q[0], q[1], q[2]};
       p[a] = wrap_index(q[a] - d, A.n);
       const int64_t j = (int64_t)p[0] * C.ax[0].stride + (int64_t)p[1] * C.ax[1].stride + p[2];
-      const double c1 = tap(row_d1<double>(A, p[a], C.inv_dx, false), d);
-      const double c2 = tap(row_d2<double>(A, p[a], C.inv_dx2), d);
-      const double c6 = C.ko * tap(row_d6<double>(A, p[a]), d);
-      const double cu = tap(row_upwind<double>(A, p[a], C.inv_dx, us[(BETA + a) * n + j]), d);
+      const double bp = us[(BETA + a) * n + j];
+      double c1, c2, c6, cu;
+      if (plain) {
+        const int k = d + 3;
+        c1 = D1_TAPS[k] * C.inv_dx;
+        c2 = D2_TAPS[k] * C.inv_dx2;
+        c6 = C.ko * D6_TAPS[k];
+        cu = (bp > 0.0 ? UPF_TAPS[k] : (bp < 0.0 ? UPB_TAPS[k] : D1_TAPS[k])) * C.inv_dx;
+      } else {
+        c1 = tap(row_d1<double>(A, p[a], C.inv_dx, false), d);
+        c2 = tap(row_d2<double>(A, p[a], C.inv_dx2), d);
+        c6 = C.ko * tap(row_d6<double>(A, p[a]), d);
+        cu = tap(row_upwind<double>(A, p[a], C.inv_dx, bp), d);
+      }
       if (c1 == 0.0 && c2 == 0.0 && c6 == 0.0 && cu == 0.0) continue;
       for (int c = 0; c < NCOMP; ++c) {
         double v = c1 * fbar[(F_D1 + 3 * c + a) * n + j] + cu * fbar[(F_UP + 3 * c + a) * n + j] +
@@ -135,13 +144,15 @@ __global__ void __launch_bounds__(128) k_vjp_transpose(Ctx C, const double *__re
       const Axis &A = C.ax[a], &B = C.ax[bx];
       if (A.reduced || B.reduced) continue;
       const int Ra = reach(A, q[a]), Rb = reach(B, q[bx]);
+      const bool plain_a = Ra == 3 && A.q == 1.0, plain_b = Rb == 3 && B.q == 1.0;
       for (int da = -Ra; da <= Ra; ++da) {
         const int pa = wrap_index(q[a] - da, A.n);
-        const double ca = tap(row_d1<double>(A, pa, C.inv_dx, false), da);
+        const double ca = plain_a ? D1_TAPS[da + 3] * C.inv_dx : tap(row_d1<double>(A, pa, C.inv_dx, false), da);
         if (ca == 0.0) continue;
         for (int db = -Rb; db <= Rb; ++db) {
           const int pb = wrap_index(q[bx] - db, B.n);
-          const double cb = tap(row_d1<double>(B, pb, C.inv_dx, false), db);
+          const double cb =
+              plain_b ? D1_TAPS[db + 3] * C.inv_dx : tap(row_d1<double>(B, pb, C.inv_dx, false), db);
           if (cb == 0.0) continue;
           int p[3] = {q[0], q[1], q[2]};
           p[a] = pa;
@@ -194,7 +205,7 @@ extern "C" {
 
 int64_t bssn_vjp_work_size(const bssn_grid *grid) {
   if (!grid || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail(BSSN_ERR_INVALID, "invalid grid");
-  return (int64_t)NFEAT * grid->nx * grid->ny * grid->nz;
+  return 2 * (int64_t)NFEAT * grid->nx * grid->ny * grid->nz;  // features and their adjoints
 }
 
 int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *state, const double *rhs_bar,
@@ -206,9 +217,11 @@ int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *stat
   if (!rhs_bar && !h_bar) return fail(BSSN_ERR_INVALID, "rhs_bar and h_bar are both null");
   const hipStream_t st = (hipStream_t)stream;
   const int64_t b64 = (C.ncell + 63) / 64, b128 = (C.ncell + 127) / 128;
-  hipLaunchKernelGGL(k_vjp_point, dim3((unsigned)b64), dim3(64), 0, st, C, state, rhs_bar, h_bar, work);
+  double *feat = work, *fbar = work + (int64_t)NFEAT * C.ncell;
+  if ((rc = launch_record_f64(C, state, feat, st))) return rc;
+  hipLaunchKernelGGL(k_vjp_point, dim3((unsigned)b64), dim3(64), 0, st, C, feat, rhs_bar, h_bar, fbar);
   if ((rc = check_hip(hipGetLastError(), "adjoint point kernel launch"))) return rc;
-  hipLaunchKernelGGL(k_vjp_transpose, dim3((unsigned)b128), dim3(128), 0, st, C, state, work, state_bar, accumulate);
+  hipLaunchKernelGGL(k_vjp_transpose, dim3((unsigned)b128), dim3(128), 0, st, C, state, fbar, state_bar, accumulate);
   return check_hip(hipGetLastError(), "adjoint transpose kernel launch");
 }
 

@@ -8,6 +8,9 @@
 
 #include <algorithm>
 
+#include <type_traits>
+
+#include "adjoint_point.h"
 #include "bssn_amd.h"
 #include "bssn_common.h"
 #include "host_errors.h"
@@ -48,12 +51,98 @@ __device__ __forceinline__ void load_point(const Ctx &C, const double *__restric
   for (int c = 0; c < NCOMP; ++c) v[c] = g[c * C.ncell + idx];
 }
 
+// ------------------------------------------------------ feature recording
+
+// A stencil provider that also stores every stencil value the per-point
+// RHS asks of it -- the features of adjoint_point.h: first, second, upwind
+// derivatives and Kreiss-Oliger sums of the visited component, and on a
+// Sommerfeld face point the first derivatives of the face RHS -- into
+// feat[slot * n + idx].  Run through the stage kernels with an emitter that
+// keeps nothing, the compiler drops the algebra after the stencils.
+template <class P>
+struct Recorder : P {
+  // (names of their own: the providers have members such as idx)
+  BSSN_GLOBAL double *rec_feat;
+  int64_t rec_n, rec_idx;
+  bool rec_on;
+  int vc;  // component of the current visit
+  __device__ __forceinline__ void record_to(double *f, int64_t ncell, int64_t i, bool on) {
+    rec_feat = (BSSN_GLOBAL double *)f;
+    rec_n = ncell;
+    rec_idx = i;
+    rec_on = on;
+  }
+  __device__ __forceinline__ void keep(int slot, double v) const {
+    if (rec_on) rec_feat[slot * rec_n + rec_idx] = v;
+  }
+  __device__ __forceinline__ void stage(int v) {
+    P::stage(v);
+    vc = visit_comp(v);
+  }
+  __device__ __forceinline__ auto d1(int a) const {
+    const auto v = P::d1(a);
+    keep(F_D1 + 3 * vc + a, (double)v);
+    return v;
+  }
+  __device__ __forceinline__ auto d2(int a) const {
+    const auto v = P::d2(a);
+    keep(F_D2 + 6 * d2_slot(vc) + sym(a, a), (double)v);
+    return v;
+  }
+  __device__ __forceinline__ auto dmix(int a, int b) const {
+    const auto v = P::dmix(a, b);
+    keep(F_D2 + 6 * d2_slot(vc) + sym(a, b), (double)v);
+    return v;
+  }
+  template <typename C>
+  __device__ __forceinline__ auto upw(int a, C coeff) const {
+    const auto v = P::upw(a, coeff);
+    keep(F_UP + 3 * vc + a, (double)v);
+    return v;
+  }
+  __device__ __forceinline__ auto ko() const {
+    const auto v = P::ko();
+    keep(F_KO + vc, (double)v);
+    return v;
+  }
+  __device__ __forceinline__ auto somm(int comp, double uc) const {
+    d1(0);
+    d1(1);
+    d1(2);
+    return P::somm(comp, uc);
+  }
+};
+
+// The emitter of the recording launches: the RHS values are not kept.
+struct RecordEmitter {
+  __device__ __forceinline__ void prefetch(int) {}
+  __device__ __forceinline__ void put(int, double) {}
+  __device__ __forceinline__ void put_face(int, double) {}
+  __device__ __forceinline__ bool has_matter() const { return false; }
+  __device__ __forceinline__ double matter(int) const { return 0.0; }
+  __device__ __forceinline__ double extra_A(int) const { return 0.0; }
+  __device__ __forceinline__ void done() {}
+};
+
+// Point values and the recorded stencils of one point (REC launches: the
+// feature array is O.rhs).
+template <typename T, class P>
+__device__ __forceinline__ void record_point(P &pv, const Ctx &C, const StageOut &O, int64_t idx, bool mine,
+                                             const double (&u)[NCOMP]) {
+  pv.record_to(O.rhs, C.ncell, idx, mine);
+  if (mine)
+    for (int c = 0; c < NCOMP; ++c) ((BSSN_GLOBAL double *)O.rhs)[(F_VAL + c) * C.ncell + idx] = u[c];
+  RecordEmitter out;
+  bssn_rhs_point<T>(pv, C, u, out);
+}
+
 // ------------------------------------------------------------ tile kernel
 
-template <typename T, int TX, int TY, int TZ, bool MAD, bool CL, int CLA = 7>
+template <typename T, int TX, int TY, int TZ, bool MAD, bool CL, int CLA = 7, bool REC = false>
 __global__ void __launch_bounds__(TX *TY *TZ, (tile_min_waves<T, CL>()))
     k_stage_tile(Ctx C, const double *__restrict__ us, StageOut O, TileSet S) {
-  using P = TileProvider<T, TX, TY, TZ, MAD, CL, CLA, sizeof(T) == sizeof(double) && !CL>;
+  using PB = TileProvider<T, TX, TY, TZ, MAD, CL, CLA, sizeof(T) == sizeof(double) && !CL>;
+  using P = std::conditional_t<REC, Recorder<PB>, PB>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   int bx, by, bz, slab;
@@ -83,8 +172,12 @@ __global__ void __launch_bounds__(TX *TY *TZ, (tile_min_waves<T, CL>()))
   load_point(C, us, idx, u);
   // without in-tile closures the shell points belong to the shell kernel
   const bool mine = inside && (CL || !(C.any_shell && in_shell(C, x, y, z)));
-  Emitter<LdsStash<TX * TY * TZ>> out(C, O, idx, mine, pv.face(), pv.st);
-  bssn_rhs_point<T>(pv, C, u, out);
+  if constexpr (REC) {
+    record_point<T>(pv, C, O, idx, mine, u);
+  } else {
+    Emitter<LdsStash<TX * TY * TZ>> out(C, O, idx, mine, pv.face(), pv.st);
+    bssn_rhs_point<T>(pv, C, u, out);
+  }
 }
 
 // ------------------------------------------------------- region tiles
@@ -167,7 +260,7 @@ __device__ __forceinline__ bool in_slab(const Slabs &S, int s, int x, int y, int
          z < S.hi[s][2];
 }
 
-template <typename T>
+template <typename T, bool REC = false>
 __global__ void __launch_bounds__(256) k_stage_shell(Ctx C, const double *__restrict__ us, StageOut O, Slabs S) {
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= S.first[S.count]) return;
@@ -181,12 +274,16 @@ __global__ void __launch_bounds__(256) k_stage_shell(Ctx C, const double *__rest
   for (int t = 0; t < s; ++t)
     if (in_slab(S, t, x, y, z)) return;
 
-  PointProvider<T> pv;
+  std::conditional_t<REC, Recorder<PointProvider<T>>, PointProvider<T>> pv;
   pv.setup(C, us, x, y, z);
   double u[NCOMP];
   load_point(C, us, pv.idx, u);
-  Emitter<RegStash> out(C, O, pv.idx, true, pv.face(), pv.st);
-  bssn_rhs_point<T>(pv, C, u, out);
+  if constexpr (REC) {
+    record_point<T>(pv, C, O, pv.idx, true, u);
+  } else {
+    Emitter<RegStash> out(C, O, pv.idx, true, pv.face(), pv.st);
+    bssn_rhs_point<T>(pv, C, u, out);
+  }
 }
 
 // Tile shape and the minimum waves per SIMD the tile kernels are compiled
@@ -215,24 +312,24 @@ inline bool tiles_aligned(const Ctx &C) {
   return true;
 }
 
-template <typename T, bool MAD, int CLA>
+template <typename T, bool MAD, int CLA, bool REC>
 inline int launch_face_tiles(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int kind) {
   const TileSet S = tile_set(C, kind, TX, TY, TZ);
   if (S.count <= 0) return BSSN_OK;
-  hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true, CLA>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
-                     O, S);
+  hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, true, CLA, REC>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C,
+                     us, O, S);
   return check_hip(hipGetLastError(), "tile kernel launch");
 }
 
-template <typename T, bool MAD, bool CL>
+template <typename T, bool MAD, bool CL, bool REC>
 inline int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
   int rc = BSSN_OK;
   // lean kernel: every tile, or with in-tile closures only the interior tiles
   if (parts & (PART_TILES | PART_LEAN)) {
     const TileSet S = tile_set(C, CL ? 1 : 0, TX, TY, TZ);
     if (S.count > 0) {
-      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false>), dim3(S.count), dim3(TX * TY * TZ), 0, st, C, us,
-                         O, S);
+      hipLaunchKernelGGL((k_stage_tile<T, TX, TY, TZ, MAD, false, 7, REC>), dim3(S.count), dim3(TX * TY * TZ), 0, st,
+                         C, us, O, S);
       rc = check_hip(hipGetLastError(), "tile kernel launch");
       if (rc) return rc;
     }
@@ -242,12 +339,12 @@ inline int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hip
   // MAD every face tile takes the general kernel)
   if (CL && (parts & (PART_TILES | PART_CLOSURE))) {
     if constexpr (MAD) {
-      rc = launch_face_tiles<T, MAD, 7>(C, us, O, st, 2);
+      rc = launch_face_tiles<T, MAD, 7, REC>(C, us, O, st, 2);
     } else {
-      if (!rc) rc = launch_face_tiles<T, MAD, 1>(C, us, O, st, 3);
-      if (!rc) rc = launch_face_tiles<T, MAD, 2>(C, us, O, st, 4);
-      if (!rc) rc = launch_face_tiles<T, MAD, 4>(C, us, O, st, 5);
-      if (!rc) rc = launch_face_tiles<T, MAD, 7>(C, us, O, st, 6);
+      if (!rc) rc = launch_face_tiles<T, MAD, 1, REC>(C, us, O, st, 3);
+      if (!rc) rc = launch_face_tiles<T, MAD, 2, REC>(C, us, O, st, 4);
+      if (!rc) rc = launch_face_tiles<T, MAD, 4, REC>(C, us, O, st, 5);
+      if (!rc) rc = launch_face_tiles<T, MAD, 7, REC>(C, us, O, st, 6);
     }
     if (rc) return rc;
   }
@@ -256,23 +353,29 @@ inline int launch_stage_t(const Ctx &C, const double *us, const StageOut &O, hip
     const int64_t total = S.first[S.count];
     if (total > 0) {
       const int64_t blocks = (total + 255) / 256;
-      hipLaunchKernelGGL((k_stage_shell<T>), dim3((unsigned)blocks), dim3(256), 0, st, C, us, O, S);
+      hipLaunchKernelGGL((k_stage_shell<T, REC>), dim3((unsigned)blocks), dim3(256), 0, st, C, us, O, S);
       rc = check_hip(hipGetLastError(), "shell kernel launch");
     }
   }
   return rc;
 }
 
-template <typename T>
+template <typename T, bool REC = false>
 inline int launch_stage_p(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts) {
   const bool mad = C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0;
   const bool cl = C.any_shell && tiles_aligned(C);
-  if (mad) return cl ? launch_stage_t<T, true, true>(C, us, O, st, parts) : launch_stage_t<T, true, false>(C, us, O, st, parts);
-  return cl ? launch_stage_t<T, false, true>(C, us, O, st, parts) : launch_stage_t<T, false, false>(C, us, O, st, parts);
+  if (mad)
+    return cl ? launch_stage_t<T, true, true, REC>(C, us, O, st, parts)
+              : launch_stage_t<T, true, false, REC>(C, us, O, st, parts);
+  return cl ? launch_stage_t<T, false, true, REC>(C, us, O, st, parts)
+            : launch_stage_t<T, false, false, REC>(C, us, O, st, parts);
 }
 
 // Launchers of one precision (defined in stage_f64.hip / stage_f32.hip).
 int launch_stage_f64(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts);
+// The FP64 stage kernels recording the features of every point into feat
+// (NFEAT * ncell doubles) instead of updating (defined in record_f64.hip).
+int launch_record_f64(const Ctx &C, const double *us, double *feat, hipStream_t st);
 int launch_stage_f32(const Ctx &C, const double *us, const StageOut &O, hipStream_t st, int parts);
 // the closure-capable point kernel over a list of slabs
 int launch_slabs_f64(const Ctx &C, const double *us, const StageOut &O, const Slabs &S, hipStream_t st);
