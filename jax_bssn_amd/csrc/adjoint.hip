@@ -35,37 +35,51 @@ __device__ __forceinline__ void point_of(const Ctx &C, int64_t i, int (&q)[3]) {
   q[0] = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
 }
 
+// Features of one point in the recorded grid functions (k-th at p[k n]),
+// and their adjoints: stored once each, or added to what is stored.
+struct StridedFeatures {
+  const double *p;
+  int64_t n;
+  __device__ __forceinline__ double operator()(int k) const { return p[k * n]; }
+};
+struct StridedAdjoints {
+  double *p;
+  int64_t n;
+  __device__ __forceinline__ void put(int k, double v) { p[k * n] = v; }
+};
+struct StridedAdd {
+  double *p;
+  int64_t n;
+  __device__ __forceinline__ void put(int k, double v) { p[k * n] += v; }
+};
+
 // The feature adjoints of every point from its recorded features (feat,
 // NFEAT grid functions written by the stage kernels in recording mode).
-__global__ void __launch_bounds__(64) k_vjp_point(Ctx C, const double *__restrict__ feat,
-                                                  const double *__restrict__ rbar, const double *__restrict__ hbar,
-                                                  double *__restrict__ fbar) {
+// The feature and adjoint pointers are not restrict-qualified on purpose:
+// after each store of an adjoint the compiler reloads the features it
+// needs again (cache hits) instead of keeping them live in registers.
+__global__ void __launch_bounds__(64) k_vjp_point(Ctx C, const double *feat, const double *__restrict__ rbar,
+                                                  const double *__restrict__ hbar, double *fbar) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= C.ncell) return;
   const int64_t n = C.ncell;
   int q[3];
   point_of(C, i, q);
   const bool face = rbar && sommerfeld_point(C, q[0], q[1], q[2]);
-  double f[NFEAT];
-  for (int k = 0; k < NFEAT; ++k) f[k] = feat[k * n + i];
-  if (!face)  // A's first derivatives are features of face points only
-    for (int k = F_D1 + 3 * AXX; k < F_D1 + 3 * (AZZ + 1); ++k) f[k] = 0.0;
-  double fb[NFEAT];
-  for (int k = 0; k < NFEAT; ++k) fb[k] = 0.0;
+  // the Sommerfeld RHS replaces the bulk one on an active face point: the
+  // bulk adjoint there carries the H part only (zero RHS adjoints)
   double rb[NCOMP];
   if (rbar)
-    for (int c = 0; c < NCOMP; ++c) rb[c] = rbar[c * n + i];
-  const double hb = hbar ? hbar[i] : 0.0;
+    for (int c = 0; c < NCOMP; ++c) rb[c] = face ? 0.0 : rbar[c * n + i];
+  StridedAdjoints out{fbar + i, n};
+  rhs_vjp_point(C, StridedFeatures{feat + i, n}, rbar ? rb : (const double *)nullptr, hbar ? hbar[i] : 0.0, out);
   if (face) {
-    // the Sommerfeld RHS replaces the bulk one on an active face point
-    double xr[3];
+    double xr[3], rf[NCOMP];
     for (int a = 0; a < 3; ++a) xr[a] = C.xmin[a] + C.dx * (double)q[a];
-    face_vjp_point(rb, xr, fb);
-    rhs_vjp_point(C, f, (const double *)nullptr, hb, fb);
-  } else {
-    rhs_vjp_point(C, f, rbar ? rb : (const double *)nullptr, hb, fb);
+    for (int c = 0; c < NCOMP; ++c) rf[c] = rbar[c * n + i];
+    StridedAdd add{fbar + i, n};
+    face_vjp_point(rf, xr, add);
   }
-  for (int k = 0; k < NFEAT; ++k) fbar[k * n + i] = fb[k];
 }
 
 // coefficient of the tap at offset d of a row (0 outside its reach)

@@ -84,64 +84,85 @@ struct FeatureProvider {
   }
 };
 
-// Adjoint of the Sommerfeld face RHS -(x . d u_c + u_c - u0_c) / r of every
-// component at a face point (xr: the point's coordinates).
+// Feature access of the point adjoint: a reader f(k) returns feature k of
+// the point, a sink out.put(k, v) receives the adjoint of feature k.  The
+// kernels read the recorded feature grid functions and write the adjoint
+// grid functions in place (strided by the number of points); the host
+// checks use plain arrays.
 template <typename T>
-BSSN_HD void face_vjp_point(const T *rbar, const T (&xr)[3], T *fbar) {
+struct ArrayFeatures {
+  const T *p;
+  BSSN_HD __forceinline__ T operator()(int k) const { return p[k]; }
+};
+
+template <typename T>
+struct ArrayAdjoints {  // adds into an array
+  T *p;
+  BSSN_HD __forceinline__ void put(int k, T v) { p[k] += v; }
+};
+
+template <typename T, typename FR>
+BSSN_HD __forceinline__ void unpack_feature(const FR &f, int base, T (&M)[3][3]) {
+  for (int c = 0; c < 6; ++c) {
+    const int i = SYM_I[c], j = SYM_J[c];
+    M[i][j] = M[j][i] = f(base + c);
+  }
+}
+
+// Adjoint of the Sommerfeld face RHS -(x . d u_c + u_c - u0_c) / r of every
+// component at a face point (xr: the point's coordinates), put into the
+// value and first-derivative slots.
+template <typename T, typename S>
+BSSN_HD void face_vjp_point(const T *rbar, const T (&xr)[3], S &out) {
   const T r = sqrt(xr[0] * xr[0] + xr[1] * xr[1] + xr[2] * xr[2]);
   const T rs = r > T(0) ? r : T(1);
   for (int c = 0; c < NCOMP; ++c) {
     const T w = -rbar[c] / rs;
-    fbar[F_VAL + c] += w;
-    for (int a = 0; a < 3; ++a) fbar[F_D1 + 3 * c + a] += w * xr[a];
+    out.put(F_VAL + c, w);
+    for (int a = 0; a < 3; ++a) out.put(F_D1 + 3 * c + a, w * xr[a]);
   }
 }
 
 // Adjoint of the bulk RHS (rbar, 21 values, may be null) and of the
-// Hamiltonian constraint (hbar) at a point with features f; the feature
-// adjoints are added into fbar.
-template <typename T>
-BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *fbar) {
+// Hamiltonian constraint (hbar) at a point with features f.  Puts every one
+// of the NFEAT feature adjoints exactly once (zero for the first
+// derivatives of A, which the bulk RHS does not read, and for the upwind
+// and dissipation features when rbar is null), each as soon as it is
+// final, so few adjoints are live at once.
+template <typename T, typename FR, typename S>
+BSSN_HD void rhs_vjp_point(const Ctx &C, const FR &f, const T *rbar, T hbar, S &out) {
   // ------------------------------------------------------------ inputs
   T g[3][3], A[3][3], G[3], b[3];
-  unpack_sym(f + F_VAL + GXX, 1, g);
-  unpack_sym(f + F_VAL + AXX, 1, A);
+  unpack_feature(f, F_VAL + GXX, g);
+  unpack_feature(f, F_VAL + AXX, A);
   for (int i = 0; i < 3; ++i) {
-    G[i] = f[F_VAL + GAM + i];
-    b[i] = f[F_VAL + BETA + i];
+    G[i] = f(F_VAL + GAM + i);
+    b[i] = f(F_VAL + BETA + i);
   }
-  const T W = f[F_VAL + CW], K = f[F_VAL + CK], al = f[F_VAL + ALPHA];
+  const T W = f(F_VAL + CW), K = f(F_VAL + CK), al = f(F_VAL + ALPHA);
   T db[3][3], dG[3][3], da[3], dW[3], dK[3], dg[3][3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      db[i][j] = f[F_D1 + 3 * (BETA + i) + j];
-      dG[i][j] = f[F_D1 + 3 * (GAM + i) + j];
+      db[i][j] = f(F_D1 + 3 * (BETA + i) + j);
+      dG[i][j] = f(F_D1 + 3 * (GAM + i) + j);
     }
   for (int k = 0; k < 3; ++k) {
-    da[k] = f[F_D1 + 3 * ALPHA + k];
-    dW[k] = f[F_D1 + 3 * CW + k];
-    dK[k] = f[F_D1 + 3 * CK + k];
+    da[k] = f(F_D1 + 3 * ALPHA + k);
+    dW[k] = f(F_D1 + 3 * CW + k);
+    dK[k] = f(F_D1 + 3 * CK + k);
   }
   for (int c = 0; c < 6; ++c)
-    for (int k = 0; k < 3; ++k) dg[SYM_I[c]][SYM_J[c]][k] = dg[SYM_J[c]][SYM_I[c]][k] = f[F_D1 + 3 * (GXX + c) + k];
-  T s[3][3][3], ddg[3][3][3][3], dda[3][3], ddW[3][3];
-  for (int i = 0; i < 3; ++i) unpack_sym(f + F_D2 + 6 * d2_slot(BETA + i), 1, s[i]);
-  for (int c = 0; c < 6; ++c) {
-    T M[3][3];
-    unpack_sym(f + F_D2 + 6 * d2_slot(GXX + c), 1, M);
-    for (int m = 0; m < 3; ++m)
-      for (int n = 0; n < 3; ++n) ddg[SYM_I[c]][SYM_J[c]][m][n] = ddg[SYM_J[c]][SYM_I[c]][m][n] = M[m][n];
-  }
-  unpack_sym(f + F_D2 + 6 * d2_slot(ALPHA), 1, dda);
-  unpack_sym(f + F_D2 + 6 * d2_slot(CW), 1, ddW);
+    for (int k = 0; k < 3; ++k) dg[SYM_I[c]][SYM_J[c]][k] = dg[SYM_J[c]][SYM_I[c]][k] = f(F_D1 + 3 * (GXX + c) + k);
+  T ddW[3][3];
+  unpack_feature(f, F_D2 + 6 * d2_slot(CW), ddW);
 
   // ----------------------------------------------------------- forward
   T gp[6], igp[6];
-  for (int c = 0; c < 6; ++c) gp[c] = f[F_VAL + GXX + c];
+  for (int c = 0; c < 6; ++c) gp[c] = g[SYM_I[c]][SYM_J[c]];
   inverse_metric(gp, igp);
   T ig[3][3];
   unpack_sym(igp, 1, ig);
-  T divb = db[0][0] + db[1][1] + db[2][2];
+  const T divb = db[0][0] + db[1][1] + db[2][2];
   T C1[3][3][3], C2[3][3][3], Q[3][3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
@@ -176,12 +197,18 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
     conW += acc * dW[k];
   }
   const T W2 = W * W;
-  T R[3][3], Hs[3][3];
+  // R_ij and Hs_ij (W^2 D_i D_j alpha) with their partial derivatives in W
+  // at fixed other inputs (RW, HW), so the reverse sweep needs neither the
+  // second derivatives nor the sums again
+  T R[3][3], Hs[3][3], RW[3][3], HW[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
+      T ddg[3][3];
+      unpack_feature(f, F_D2 + 6 * d2_slot(GXX + sym(i, j)), ddg);
+      const T dda = f(F_D2 + 6 * d2_slot(ALPHA) + sym(i, j));
       T r1 = T(0);
       for (int m = 0; m < 3; ++m)
-        for (int n = 0; n < 3; ++n) r1 += ig[m][n] * ddg[i][j][m][n];
+        for (int n = 0; n < 3; ++n) r1 += ig[m][n] * ddg[m][n];
       T r3 = T(0), r4 = T(0), r6 = T(0), r7 = T(0), h3 = T(0);
       for (int m = 0; m < 3; ++m) {
         r3 += g[m][i] * dG[m][j] + g[m][j] * dG[m][i];
@@ -193,7 +220,10 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
         for (int n = 0; n < 3; ++n) r7 += Q[k][n][i] * (C1[j][k][n] + C1[k][j][n]) + Q[k][n][j] * C1[i][k][n];
       R[i][j] = T(-0.5) * W2 * r1 + W * (ddW[i][j] + g[i][j] * lap) + T(0.5) * W2 * r3 + T(0.5) * W2 * r4 -
                 T(2) * g[i][j] * gradW2 - W * r6 + W2 * r7 - W * g[i][j] * conW;
-      Hs[i][j] = W2 * dda[i][j] + W * (dW[i] * da[j] + dW[j] * da[i] - g[i][j] * gWa) - W2 * h3;
+      RW[i][j] = -W * r1 + ddW[i][j] + g[i][j] * lap + W * r3 + W * r4 - r6 + T(2) * W * r7 - g[i][j] * conW;
+      const T dd = dW[i] * da[j] + dW[j] * da[i] - g[i][j] * gWa;
+      Hs[i][j] = W2 * dda + W * dd - W2 * h3;
+      HW[i][j] = T(2) * W * dda + dd - T(2) * W * h3;
     }
   T Aup[3][3], gA[3][3];
   for (int i = 0; i < 3; ++i)
@@ -223,175 +253,204 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
 
   // ----------------------------------------------------------- adjoints
   T g_[3][3] = {}, A_[3][3] = {}, ig_[3][3] = {}, R_[3][3] = {}, Hs_[3][3] = {}, Aup_[3][3] = {};
-  T C1_[3][3][3] = {}, C2_[3][3][3] = {}, Q_[3][3][3] = {}, s_[3][3][3] = {}, ddg_[3][3][3][3] = {};
-  T db_[3][3] = {}, dG_[3][3] = {}, dg_[3][3][3] = {}, dda_[3][3] = {}, ddW_[3][3] = {};
-  T G_[3] = {}, b_[3] = {}, da_[3] = {}, dW_[3] = {}, dK_[3] = {};
-  T W_ = T(0), K_ = T(0), al_ = T(0), divb_ = T(0);
-
-  if (rbar) {
-    // full output adjoints: the stored upper triangle of each symmetric RHS
-    T rA[3][3] = {}, rg[3][3] = {};
-    for (int c = 0; c < 6; ++c) {
-      rA[SYM_I[c]][SYM_J[c]] = rbar[AXX + c];
-      rg[SYM_I[c]][SYM_J[c]] = rbar[GXX + c];
-    }
-    const T rK = rbar[CK], rW = rbar[CW], ra = rbar[ALPHA];
-    T rGam[3], rb[3];
-    for (int i = 0; i < 3; ++i) {
-      rGam[i] = rbar[GAM + i];
-      rb[i] = C.zero_shift ? T(0) : rbar[BETA + i];
-    }
-    // advection and dissipation of every component: adv_c = sum_a b_a U_ca
-    for (int c = 0; c < NCOMP; ++c) {
-      const T rc = (c >= BETA && C.zero_shift) ? T(0) : rbar[c];
-      for (int a = 0; a < 3; ++a) {
-        fbar[F_UP + 3 * c + a] += rc * b[a];
-        b_[a] += rc * f[F_UP + 3 * c + a];
+  T C1_[3][3][3] = {}, C2_[3][3][3] = {};
+  T dG_[3][3] = {}, ddW_[3][3] = {};
+  T G_[3] = {}, da_[3] = {}, dW_[3] = {};
+  T W_ = T(0), K_ = T(0), al_ = T(0);
+  T rGam[3] = {T(0), T(0), T(0)};
+  {
+    T db_[3][3] = {}, b_[3] = {}, dK_[3] = {};
+    T divb_ = T(0);
+    if (rbar) {
+      // full output adjoints: the stored upper triangle of each symmetric RHS
+      T rA[3][3] = {}, rg[3][3] = {};
+      for (int c = 0; c < 6; ++c) {
+        rA[SYM_I[c]][SYM_J[c]] = rbar[AXX + c];
+        rg[SYM_I[c]][SYM_J[c]] = rbar[GXX + c];
       }
-      fbar[F_KO + c] += rc;
-    }
-    // A: al K A - 2 al A gA + Xr - g trX / 3 + ALIE
-    T Xr_[3][3], AgA_[3][3], trX_ = T(0), ALIE_[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        T AgA = T(0);
-        for (int l = 0; l < 3; ++l) AgA += A[i][l] * gA[l][j];
-        al_ += rA[i][j] * (K * A[i][j] - T(2) * AgA);
-        K_ += al * rA[i][j] * A[i][j];
-        A_[i][j] += al * K * rA[i][j];
-        AgA_[i][j] = T(-2) * al * rA[i][j];
-        Xr_[i][j] = rA[i][j];
-        g_[i][j] += T(-1.0 / 3.0) * trX * rA[i][j];
-        trX_ += T(-1.0 / 3.0) * g[i][j] * rA[i][j];
-        ALIE_[i][j] = rA[i][j];
+      const T rK = rbar[CK], rW = rbar[CW], ra = rbar[ALPHA];
+      T rb[3];
+      for (int i = 0; i < 3; ++i) {
+        rGam[i] = rbar[GAM + i];
+        rb[i] = C.zero_shift ? T(0) : rbar[BETA + i];
       }
-    {
-      T gA_[3][3] = {};
+      // advection and dissipation of every component: adv_c = sum_a b_a U_ca
+      for (int c = 0; c < NCOMP; ++c) {
+        const T rc = (c >= BETA && C.zero_shift) ? T(0) : rbar[c];
+        T up[3];
+        for (int a = 0; a < 3; ++a) up[a] = f(F_UP + 3 * c + a);
+        for (int a = 0; a < 3; ++a) {
+          out.put(F_UP + 3 * c + a, rc * b[a]);
+          b_[a] += rc * up[a];
+        }
+        out.put(F_KO + c, rc);
+      }
+      // A: al K A - 2 al A gA + Xr - g trX / 3 + ALIE
+      T Xr_[3][3], AgA_[3][3], trX_ = T(0), ALIE_[3][3];
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-          for (int l = 0; l < 3; ++l) {
-            A_[i][l] += AgA_[i][j] * gA[l][j];
-            gA_[l][j] += A[i][l] * AgA_[i][j];
-          }
-      for (int k = 0; k < 3; ++k)
-        for (int j = 0; j < 3; ++j)
-          for (int l = 0; l < 3; ++l) {
-            ig_[k][l] += gA_[k][j] * A[l][j];
-            A_[l][j] += ig[k][l] * gA_[k][j];
-          }
-    }
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        ig_[i][j] += trX_ * Xr[i][j];
-        Xr_[i][j] += trX_ * ig[i][j];
-        al_ += Xr_[i][j] * R[i][j];
-        R_[i][j] += al * Xr_[i][j];
-        Hs_[i][j] -= Xr_[i][j];
-      }
-    // ALIE = A_mi db_mj + A_mj db_mi - 2/3 A divb
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        for (int m = 0; m < 3; ++m) {
-          A_[m][i] += ALIE_[i][j] * db[m][j];
-          db_[m][j] += ALIE_[i][j] * A[m][i];
-          A_[m][j] += ALIE_[i][j] * db[m][i];
-          db_[m][i] += ALIE_[i][j] * A[m][j];
+        for (int j = 0; j < 3; ++j) {
+          T AgA = T(0);
+          for (int l = 0; l < 3; ++l) AgA += A[i][l] * gA[l][j];
+          al_ += rA[i][j] * (K * A[i][j] - T(2) * AgA);
+          K_ += al * rA[i][j] * A[i][j];
+          A_[i][j] += al * K * rA[i][j];
+          AgA_[i][j] = T(-2) * al * rA[i][j];
+          Xr_[i][j] = rA[i][j];
+          g_[i][j] += T(-1.0 / 3.0) * trX * rA[i][j];
+          trX_ += T(-1.0 / 3.0) * g[i][j] * rA[i][j];
+          ALIE_[i][j] = rA[i][j];
         }
-        A_[i][j] += T(-2.0 / 3.0) * divb * ALIE_[i][j];
-        divb_ += T(-2.0 / 3.0) * A[i][j] * ALIE_[i][j];
+      {
+        T gA_[3][3] = {};
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            for (int l = 0; l < 3; ++l) {
+              A_[i][l] += AgA_[i][j] * gA[l][j];
+              gA_[l][j] += A[i][l] * AgA_[i][j];
+            }
+        for (int k = 0; k < 3; ++k)
+          for (int j = 0; j < 3; ++j)
+            for (int l = 0; l < 3; ++l) {
+              ig_[k][l] += gA_[k][j] * A[l][j];
+              A_[l][j] += ig[k][l] * gA_[k][j];
+            }
       }
-    // K: - ig:Hs + al A:Aup + al K^2 / 3
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        ig_[i][j] -= rK * Hs[i][j];
-        Hs_[i][j] -= rK * ig[i][j];
-        al_ += rK * A[i][j] * Aup[i][j];
-        A_[i][j] += rK * al * Aup[i][j];
-        Aup_[i][j] += rK * al * A[i][j];
-      }
-    al_ += rK * K * K * T(1.0 / 3.0);
-    K_ += rK * al * K * T(2.0 / 3.0);
-    // connection: rG + 2 al ig s1 - 6 al / Wf Aup dW - 2 Aup da
-    T s1_[3] = {};
-    T Wf_ = T(0);
-    for (int i = 0; i < 3; ++i) {
-      T t2 = T(0), t3 = T(0);
-      for (int j = 0; j < 3; ++j) {
-        t2 += ig[i][j] * s1[j];
-        t3 += Aup[i][j] * dW[j];
-        ig_[i][j] += rGam[i] * T(2) * al * s1[j];
-        s1_[j] += rGam[i] * T(2) * al * ig[i][j];
-        Aup_[i][j] += rGam[i] * (T(-6) * al / Wf * dW[j] - T(2) * da[j]);
-        dW_[j] += rGam[i] * T(-6) * al / Wf * Aup[i][j];
-        da_[j] += rGam[i] * T(-2) * Aup[i][j];
-      }
-      al_ += rGam[i] * (T(2) * t2 - T(6) / Wf * t3);
-      Wf_ += rGam[i] * T(6) * al / (Wf * Wf) * t3;
-    }
-    if (W > T(1.0e-12)) W_ += Wf_;
-    for (int m = 0; m < 3; ++m)
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) {
-          C1_[m][j][k] += s1_[m] * Aup[j][k];
-          Aup_[j][k] += s1_[m] * C1[m][j][k];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          ig_[i][j] += trX_ * Xr[i][j];
+          Xr_[i][j] += trX_ * ig[i][j];
+          al_ += Xr_[i][j] * R[i][j];
+          R_[i][j] += al * Xr_[i][j];
+          Hs_[i][j] -= Xr_[i][j];
         }
-    // rG: ig:s_i + 1/3 ig_im s_jmj + 2/3 G divb - G_j db_ij - 4/3 al ig dK
-    for (int i = 0; i < 3; ++i) {
-      const T r = rGam[i];
+      // ALIE = A_mi db_mj + A_mj db_mi - 2/3 A divb
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          for (int m = 0; m < 3; ++m) {
+            A_[m][i] += ALIE_[i][j] * db[m][j];
+            db_[m][j] += ALIE_[i][j] * A[m][i];
+            A_[m][j] += ALIE_[i][j] * db[m][i];
+            db_[m][i] += ALIE_[i][j] * A[m][j];
+          }
+          A_[i][j] += T(-2.0 / 3.0) * divb * ALIE_[i][j];
+          divb_ += T(-2.0 / 3.0) * A[i][j] * ALIE_[i][j];
+        }
+      // K: - ig:Hs + al A:Aup + al K^2 / 3
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          ig_[i][j] -= rK * Hs[i][j];
+          Hs_[i][j] -= rK * ig[i][j];
+          al_ += rK * A[i][j] * Aup[i][j];
+          A_[i][j] += rK * al * Aup[i][j];
+          Aup_[i][j] += rK * al * A[i][j];
+        }
+      al_ += rK * K * K * T(1.0 / 3.0);
+      K_ += rK * al * K * T(2.0 / 3.0);
+      // connection: rG + 2 al ig s1 - 6 al / Wf Aup dW - 2 Aup da
+      T s1_[3] = {};
+      T Wf_ = T(0);
+      for (int i = 0; i < 3; ++i) {
+        T t2 = T(0), t3 = T(0);
+        for (int j = 0; j < 3; ++j) {
+          t2 += ig[i][j] * s1[j];
+          t3 += Aup[i][j] * dW[j];
+          ig_[i][j] += rGam[i] * T(2) * al * s1[j];
+          s1_[j] += rGam[i] * T(2) * al * ig[i][j];
+          Aup_[i][j] += rGam[i] * (T(-6) * al / Wf * dW[j] - T(2) * da[j]);
+          dW_[j] += rGam[i] * T(-6) * al / Wf * Aup[i][j];
+          da_[j] += rGam[i] * T(-2) * Aup[i][j];
+        }
+        al_ += rGam[i] * (T(2) * t2 - T(6) / Wf * t3);
+        Wf_ += rGam[i] * T(6) * al / (Wf * Wf) * t3;
+      }
+      if (W > T(1.0e-12)) W_ += Wf_;
       for (int m = 0; m < 3; ++m)
-        for (int n = 0; n < 3; ++n) {
-          s_[i][m][n] += r * ig[m][n];
-          ig_[m][n] += r * s[i][m][n];
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) {
+            C1_[m][j][k] += s1_[m] * Aup[j][k];
+            Aup_[j][k] += s1_[m] * C1[m][j][k];
+          }
+      // rG: ig:s_i + 1/3 ig_im s_jmj + 2/3 G divb - G_j db_ij - 4/3 al ig dK
+      // (the adjoints of the shift's second derivatives s are put below)
+      for (int i = 0; i < 3; ++i) {
+        const T r = rGam[i];
+        T s[3][3];
+        unpack_feature(f, F_D2 + 6 * d2_slot(BETA + i), s);
+        for (int m = 0; m < 3; ++m)
+          for (int n = 0; n < 3; ++n) ig_[m][n] += r * s[m][n];
+        for (int j = 0; j < 3; ++j)
+          for (int m = 0; m < 3; ++m) ig_[i][m] += T(1.0 / 3.0) * r * f(F_D2 + 6 * d2_slot(BETA + j) + sym(m, j));
+        G_[i] += r * T(2.0 / 3.0) * divb;
+        divb_ += r * T(2.0 / 3.0) * G[i];
+        for (int j = 0; j < 3; ++j) {
+          G_[j] -= r * db[i][j];
+          db_[i][j] -= r * G[j];
+          ig_[i][j] += T(-4.0 / 3.0) * al * r * dK[j];
+          dK_[j] += T(-4.0 / 3.0) * al * r * ig[i][j];
+          al_ += T(-4.0 / 3.0) * r * ig[i][j] * dK[j];
         }
-      for (int j = 0; j < 3; ++j)
-        for (int m = 0; m < 3; ++m) {
-          ig_[i][m] += T(1.0 / 3.0) * r * s[j][m][j];
-          s_[j][m][j] += T(1.0 / 3.0) * r * ig[i][m];
-        }
-      G_[i] += r * T(2.0 / 3.0) * divb;
-      divb_ += r * T(2.0 / 3.0) * G[i];
-      for (int j = 0; j < 3; ++j) {
-        G_[j] -= r * db[i][j];
-        db_[i][j] -= r * G[j];
-        ig_[i][j] += T(-4.0 / 3.0) * al * r * dK[j];
-        dK_[j] += T(-4.0 / 3.0) * al * r * ig[i][j];
-        al_ += T(-4.0 / 3.0) * r * ig[i][j] * dK[j];
       }
-    }
-    // W: al W K / 3 - W divb / 3
-    al_ += rW * W * K * T(1.0 / 3.0);
-    W_ += rW * (al * K - divb) * T(1.0 / 3.0);
-    K_ += rW * al * W * T(1.0 / 3.0);
-    divb_ -= rW * W * T(1.0 / 3.0);
-    // lapse: -al^2 K (harmonic) or -2 al K (1+log)
-    if (C.gauge == 0) {
-      al_ += ra * T(-2) * al * K;
-      K_ += ra * (-(al * al));
+      // W: al W K / 3 - W divb / 3
+      al_ += rW * W * K * T(1.0 / 3.0);
+      W_ += rW * (al * K - divb) * T(1.0 / 3.0);
+      K_ += rW * al * W * T(1.0 / 3.0);
+      divb_ -= rW * W * T(1.0 / 3.0);
+      // lapse: -al^2 K (harmonic) or -2 al K (1+log)
+      if (C.gauge == 0) {
+        al_ += ra * T(-2) * al * K;
+        K_ += ra * (-(al * al));
+      } else {
+        al_ += ra * T(-2) * K;
+        K_ += ra * T(-2) * al;
+      }
+      // metric: g_mi db_mj + g_mj db_mi - 2/3 g divb - 2 al A
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const T r = rg[i][j];
+          for (int m = 0; m < 3; ++m) {
+            g_[m][i] += r * db[m][j];
+            db_[m][j] += r * g[m][i];
+            g_[m][j] += r * db[m][i];
+            db_[m][i] += r * g[m][j];
+          }
+          g_[i][j] += T(-2.0 / 3.0) * divb * r;
+          divb_ += T(-2.0 / 3.0) * g[i][j] * r;
+          al_ += T(-2) * A[i][j] * r;
+          A_[i][j] += T(-2) * al * r;
+        }
+      // shift: g G - eta b (when evolved)
+      for (int i = 0; i < 3; ++i) {
+        G_[i] += rb[i] * (T)C.g;
+        b_[i] -= rb[i] * (T)C.eta;
+      }
     } else {
-      al_ += ra * T(-2) * K;
-      K_ += ra * T(-2) * al;
-    }
-    // metric: g_mi db_mj + g_mj db_mi - 2/3 g divb - 2 al A
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        const T r = rg[i][j];
-        for (int m = 0; m < 3; ++m) {
-          g_[m][i] += r * db[m][j];
-          db_[m][j] += r * g[m][i];
-          g_[m][j] += r * db[m][i];
-          db_[m][i] += r * g[m][j];
-        }
-        g_[i][j] += T(-2.0 / 3.0) * divb * r;
-        divb_ += T(-2.0 / 3.0) * g[i][j] * r;
-        al_ += T(-2) * A[i][j] * r;
-        A_[i][j] += T(-2) * al * r;
+      for (int c = 0; c < NCOMP; ++c) {
+        for (int a = 0; a < 3; ++a) out.put(F_UP + 3 * c + a, T(0));
+        out.put(F_KO + c, T(0));
       }
-    // shift: g G - eta b (when evolved)
+    }
+    // the shift, its first derivatives and those of K are read by the RHS alone
+    for (int i = 0; i < 3; ++i) db_[i][i] += divb_;
     for (int i = 0; i < 3; ++i) {
-      G_[i] += rb[i] * (T)C.g;
-      b_[i] -= rb[i] * (T)C.eta;
+      out.put(F_VAL + BETA + i, b_[i]);
+      out.put(F_D1 + 3 * CK + i, dK_[i]);
+      for (int j = 0; j < 3; ++j) out.put(F_D1 + 3 * (BETA + i) + j, db_[i][j]);
     }
   }
+  // shift second derivatives: s_j,mn = rG_j ig^mn + delta_nj ig^im rG_i / 3
+  {
+    T v[3];
+    for (int m = 0; m < 3; ++m) v[m] = T(1.0 / 3.0) * (rGam[0] * ig[0][m] + rGam[1] * ig[1][m] + rGam[2] * ig[2][m]);
+    for (int j = 0; j < 3; ++j) {
+      T sb[3][3];
+      for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n) sb[m][n] = rGam[j] * ig[m][n] + (n == j ? v[m] : T(0));
+      for (int p = 0; p < 6; ++p) out.put(F_D2 + 6 * d2_slot(BETA + j) + p, fold_sym(sb, p));
+    }
+  }
+  // first derivatives of A are features of Sommerfeld face points only
+  for (int c = 0; c < 6; ++c)
+    for (int k = 0; k < 3; ++k) out.put(F_D1 + 3 * (AXX + c) + k, T(0));
 
   // Hamiltonian constraint: ig:R + 2/3 K^2 - A:Aup
   if (hbar != T(0)) {
@@ -404,6 +463,8 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
       }
     K_ += hbar * T(4.0 / 3.0) * K;
   }
+  out.put(F_VAL + CK, K_);
+  out.put(F_VAL + ALPHA, al_);
 
   // Aup = ig A ig
   for (int i = 0; i < 3; ++i)
@@ -415,16 +476,14 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
           ig_[j][l] += w * ig[i][k] * A[k][l];
           A_[k][l] += w * ig[i][k] * ig[j][l];
         }
+  for (int c = 0; c < 6; ++c) out.put(F_VAL + AXX + c, fold_sym(A_, c));
 
   // Hs = W^2 dda + W (dW da + da dW - g gWa) - W^2 C2^k da_k
   T gWa_ = T(0);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       const T h = Hs_[i][j];
-      T h3 = T(0);
-      for (int k = 0; k < 3; ++k) h3 += C2[k][i][j] * da[k];
-      W_ += h * (T(2) * W * dda[i][j] + (dW[i] * da[j] + dW[j] * da[i] - g[i][j] * gWa) - T(2) * W * h3);
-      dda_[i][j] += W2 * h;
+      W_ += h * HW[i][j];
       dW_[i] += W * h * da[j];
       da_[j] += W * h * dW[i];
       dW_[j] += W * h * da[i];
@@ -436,58 +495,78 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
         da_[k] -= W2 * h * C2[k][i][j];
       }
     }
+  for (int p = 0; p < 6; ++p) out.put(F_D2 + 6 * d2_slot(ALPHA) + p, W2 * fold_sym(Hs_, p));
+  for (int m = 0; m < 3; ++m)
+    for (int n = 0; n < 3; ++n) {
+      ig_[m][n] += gWa_ * dW[m] * da[n];
+      dW_[m] += gWa_ * ig[m][n] * da[n];
+      da_[n] += gWa_ * ig[m][n] * dW[m];
+    }
+  for (int k = 0; k < 3; ++k) out.put(F_D1 + 3 * ALPHA + k, da_[k]);
 
   // R
   T lap_ = T(0), gradW2_ = T(0), conW_ = T(0);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       const T r = R_[i][j];
-      T r1 = T(0), r3 = T(0), r4 = T(0), r6 = T(0), r7 = T(0);
-      for (int m = 0; m < 3; ++m)
-        for (int n = 0; n < 3; ++n) {
-          r1 += ig[m][n] * ddg[i][j][m][n];
-          ig_[m][n] += T(-0.5) * W2 * r * ddg[i][j][m][n];
-          ddg_[i][j][m][n] += T(-0.5) * W2 * r * ig[m][n];
-        }
       for (int m = 0; m < 3; ++m) {
-        r3 += g[m][i] * dG[m][j] + g[m][j] * dG[m][i];
         g_[m][i] += T(0.5) * W2 * r * dG[m][j];
         dG_[m][j] += T(0.5) * W2 * r * g[m][i];
         g_[m][j] += T(0.5) * W2 * r * dG[m][i];
         dG_[m][i] += T(0.5) * W2 * r * g[m][j];
-        r4 += G[m] * (C1[i][j][m] + C1[j][i][m]);
         G_[m] += T(0.5) * W2 * r * (C1[i][j][m] + C1[j][i][m]);
         C1_[i][j][m] += T(0.5) * W2 * r * G[m];
         C1_[j][i][m] += T(0.5) * W2 * r * G[m];
-        r6 += C2[m][i][j] * dW[m];
         C2_[m][i][j] -= W * r * dW[m];
         dW_[m] -= W * r * C2[m][i][j];
       }
       for (int k = 0; k < 3; ++k)
         for (int n = 0; n < 3; ++n) {
-          r7 += Q[k][n][i] * (C1[j][k][n] + C1[k][j][n]) + Q[k][n][j] * C1[i][k][n];
-          Q_[k][n][i] += W2 * r * (C1[j][k][n] + C1[k][j][n]);
           C1_[j][k][n] += W2 * r * Q[k][n][i];
           C1_[k][j][n] += W2 * r * Q[k][n][i];
-          Q_[k][n][j] += W2 * r * C1[i][k][n];
           C1_[i][k][n] += W2 * r * Q[k][n][j];
         }
-      // W-dependence of every term
-      W_ += r * (-W * r1 + ddW[i][j] + g[i][j] * lap + W * r3 + W * r4 - r6 + T(2) * W * r7 - g[i][j] * conW);
+      W_ += r * RW[i][j];
       ddW_[i][j] += W * r;
       g_[i][j] += r * (W * lap - T(2) * gradW2 - W * conW);
       lap_ += W * r * g[i][j];
       gradW2_ += T(-2) * r * g[i][j];
       conW_ += -W * r * g[i][j];
     }
+  for (int i = 0; i < 3; ++i) {
+    out.put(F_VAL + GAM + i, G_[i]);
+    for (int j = 0; j < 3; ++j) out.put(F_D1 + 3 * (GAM + i) + j, dG_[i][j]);
+  }
+  out.put(F_VAL + CW, W_);
+  // the metric's second derivatives enter R_ij as -W^2/2 ig^mn d_m d_n g_ij
+  for (int c = 0; c < 6; ++c) {
+    const T w = T(-0.5) * W2 * fold_sym(R_, c);
+    T M[3][3];
+    unpack_feature(f, F_D2 + 6 * d2_slot(GXX + c), M);
+    for (int m = 0; m < 3; ++m)
+      for (int n = 0; n < 3; ++n) ig_[m][n] += w * M[m][n];
+    for (int p = 0; p < 6; ++p) out.put(F_D2 + 6 * d2_slot(GXX + c) + p, w * fold_sym(ig, p));
+  }
+  // Q^k_n^i = ig^mn C2^k_mi, whose adjoint is formed one entry at a time
+  for (int k = 0; k < 3; ++k)
+    for (int n = 0; n < 3; ++n)
+      for (int x = 0; x < 3; ++x) {
+        T q = T(0);
+        for (int j = 0; j < 3; ++j) q += R_[x][j] * (C1[j][k][n] + C1[k][j][n]) + R_[j][x] * C1[j][k][n];
+        q *= W2;
+        for (int m = 0; m < 3; ++m) {
+          ig_[m][n] += q * C2[k][m][x];
+          C2_[k][m][x] += q * ig[m][n];
+        }
+      }
   for (int m = 0; m < 3; ++m)
     for (int n = 0; n < 3; ++n) {
-      ig_[m][n] += lap_ * ddW[m][n] + gradW2_ * dW[m] * dW[n] + gWa_ * dW[m] * da[n];
+      ig_[m][n] += lap_ * ddW[m][n] + gradW2_ * dW[m] * dW[n];
       ddW_[m][n] += lap_ * ig[m][n];
-      dW_[m] += gradW2_ * ig[m][n] * dW[n] + gWa_ * ig[m][n] * da[n];
+      dW_[m] += gradW2_ * ig[m][n] * dW[n];
       dW_[n] += gradW2_ * ig[m][n] * dW[m];
-      da_[n] += gWa_ * ig[m][n] * dW[m];
     }
+  for (int p = 0; p < 6; ++p) out.put(F_D2 + 6 * d2_slot(CW) + p, fold_sym(ddW_, p));
   // conW = sum_k (ig:C2^k) dW_k
   for (int k = 0; k < 3; ++k) {
     dW_[k] += conW_ * trC2[k];
@@ -497,14 +576,7 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
         C2_[k][m][n] += conW_ * ig[m][n] * dW[k];
       }
   }
-  // Q^k_n^i = ig^mn C2^k_mi
-  for (int k = 0; k < 3; ++k)
-    for (int n = 0; n < 3; ++n)
-      for (int i = 0; i < 3; ++i)
-        for (int m = 0; m < 3; ++m) {
-          ig_[m][n] += Q_[k][n][i] * C2[k][m][i];
-          C2_[k][m][i] += Q_[k][n][i] * ig[m][n];
-        }
+  for (int k = 0; k < 3; ++k) out.put(F_D1 + 3 * CW + k, dW_[k]);
   // C2^k_ij = ig^kl C1_lij
   for (int k = 0; k < 3; ++k)
     for (int i = 0; i < 3; ++i)
@@ -514,16 +586,21 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
           C1_[l][i][j] += C2_[k][i][j] * ig[k][l];
         }
   // C1_ijk = (d_k g_ij + d_j g_ik - d_i g_jk) / 2
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      for (int k = 0; k < 3; ++k) {
-        const T c = T(0.5) * C1_[i][j][k];
-        dg_[i][j][k] += c;
-        dg_[i][k][j] += c;
-        dg_[j][k][i] -= c;
-      }
-  // divb
-  for (int i = 0; i < 3; ++i) db_[i][i] += divb_;
+  {
+    T dg_[3][3][3] = {};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          const T c = T(0.5) * C1_[i][j][k];
+          dg_[i][j][k] += c;
+          dg_[i][k][j] += c;
+          dg_[j][k][i] -= c;
+        }
+    for (int c = 0; c < 6; ++c) {
+      const int i = SYM_I[c], j = SYM_J[c];
+      for (int k = 0; k < 3; ++k) out.put(F_D1 + 3 * (GXX + c) + k, i == j ? dg_[i][i][k] : dg_[i][j][k] + dg_[j][i][k]);
+    }
+  }
   // ig = g^-1: g_ -= ig ig_ ig
   for (int a = 0; a < 3; ++a)
     for (int bb = 0; bb < 3; ++bb) {
@@ -532,40 +609,7 @@ BSSN_HD void rhs_vjp_point(const Ctx &C, const T *f, const T *rbar, T hbar, T *f
         for (int d = 0; d < 3; ++d) acc += ig[a][c] * ig_[c][d] * ig[d][bb];
       g_[a][bb] -= acc;
     }
-
-  // ------------------------------------------------- fold into features
-  for (int c = 0; c < 6; ++c) {
-    fbar[F_VAL + GXX + c] += fold_sym(g_, c);
-    fbar[F_VAL + AXX + c] += fold_sym(A_, c);
-  }
-  fbar[F_VAL + CW] += W_;
-  fbar[F_VAL + CK] += K_;
-  fbar[F_VAL + ALPHA] += al_;
-  for (int i = 0; i < 3; ++i) {
-    fbar[F_VAL + GAM + i] += G_[i];
-    fbar[F_VAL + BETA + i] += b_[i];
-    for (int j = 0; j < 3; ++j) {
-      fbar[F_D1 + 3 * (BETA + i) + j] += db_[i][j];
-      fbar[F_D1 + 3 * (GAM + i) + j] += dG_[i][j];
-    }
-    fbar[F_D1 + 3 * ALPHA + i] += da_[i];
-    fbar[F_D1 + 3 * CW + i] += dW_[i];
-    fbar[F_D1 + 3 * CK + i] += dK_[i];
-  }
-  for (int c = 0; c < 6; ++c) {
-    const int i = SYM_I[c], j = SYM_J[c];
-    for (int k = 0; k < 3; ++k) fbar[F_D1 + 3 * (GXX + c) + k] += i == j ? dg_[i][i][k] : dg_[i][j][k] + dg_[j][i][k];
-    T M[3][3];
-    for (int m = 0; m < 3; ++m)
-      for (int n = 0; n < 3; ++n) M[m][n] = i == j ? ddg_[i][i][m][n] : ddg_[i][j][m][n] + ddg_[j][i][m][n];
-    for (int p = 0; p < 6; ++p) fbar[F_D2 + 6 * d2_slot(GXX + c) + p] += fold_sym(M, p);
-  }
-  for (int i = 0; i < 3; ++i)
-    for (int p = 0; p < 6; ++p) fbar[F_D2 + 6 * d2_slot(BETA + i) + p] += fold_sym(s_[i], p);
-  for (int p = 0; p < 6; ++p) {
-    fbar[F_D2 + 6 * d2_slot(ALPHA) + p] += fold_sym(dda_, p);
-    fbar[F_D2 + 6 * d2_slot(CW) + p] += fold_sym(ddW_, p);
-  }
+  for (int c = 0; c < 6; ++c) out.put(F_VAL + GXX + c, fold_sym(g_, c));
 }
 
 // Adjoint of the algebraic projection P (reference time_evolve.py:94-131:

@@ -120,7 +120,19 @@ extern "C" int host_features_forward(const bssn_params *p, const bssn_grid *g, c
   return 0;
 }
 
+// Sink that adds into an array and counts the puts of every slot.
+struct CountingAdjoints {
+  double *p;
+  int count[NFEAT];
+  void put(int k, double v) {
+    p[k] += v;
+    ++count[k];
+  }
+};
+
 // The feature adjoints (fbar, NFEAT) of <rbar, RHS> + hbar H at a point.
+// Returns 2 when the bulk adjoint does not put every feature adjoint
+// exactly once (the kernels store rather than add them).
 extern "C" int host_features_vjp(const bssn_params *p, const bssn_grid *g, const double *f, int face,
                                  const double *xr, const double *rbar, double hbar, double *fbar, char *err,
                                  int errlen) {
@@ -128,12 +140,18 @@ extern "C" int host_features_vjp(const bssn_params *p, const bssn_grid *g, const
   const int rc = resolve(p, g, &C, err, (size_t)errlen);
   if (rc) return rc;
   for (int k = 0; k < NFEAT; ++k) fbar[k] = 0.0;
+  CountingAdjoints out{fbar, {}};
+  const ArrayFeatures<double> fr{f};
+  rhs_vjp_point(C, fr, face ? (const double *)nullptr : rbar, hbar, out);
+  for (int k = 0; k < NFEAT; ++k)
+    if (out.count[k] != 1) {
+      snprintf(err, (size_t)errlen, "feature adjoint %d put %d times", k, out.count[k]);
+      return 2;
+    }
   if (face) {
     double x[3] = {xr[0], xr[1], xr[2]};
-    face_vjp_point(rbar, x, fbar);
-    rhs_vjp_point(C, f, (const double *)nullptr, hbar, fbar);
-  } else {
-    rhs_vjp_point(C, f, rbar, hbar, fbar);
+    ArrayAdjoints<double> add{fbar};
+    face_vjp_point(rbar, x, add);
   }
   return 0;
 }
