@@ -107,14 +107,27 @@ __device__ __forceinline__ int reach(const Axis &A, int qa) {
   return r < A.n ? r : A.n - 1;
 }
 
-__global__ void __launch_bounds__(128) k_vjp_transpose(Ctx C, const double *__restrict__ us,
-                                                       const double *__restrict__ fbar, double *__restrict__ ubar,
-                                                       int accumulate) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.ncell) return;
+// Points of the transpose in 4 x 4 x 16 tiles (x, y, z; z fastest over
+// the lanes), one tile per 256-thread workgroup, tiles in x-major order
+// spread over the XCDs in contiguous ranges: the taps of neighbouring rows
+// and planes are read by the same XCD, from its L2.
+constexpr int TT_X = 4, TT_Y = 4, TT_Z = 16, TT_THREADS = TT_X * TT_Y * TT_Z;
+
+__host__ __device__ inline int64_t transpose_tiles(const Ctx &C) {
+  return (int64_t)((C.ax[0].n + TT_X - 1) / TT_X) * ((C.ax[1].n + TT_Y - 1) / TT_Y) * ((C.ax[2].n + TT_Z - 1) / TT_Z);
+}
+
+__global__ void __launch_bounds__(TT_THREADS) k_vjp_transpose(Ctx C, const double *__restrict__ us,
+                                                              const double *__restrict__ fbar,
+                                                              double *__restrict__ ubar, int accumulate) {
+  const int nty = (C.ax[1].n + TT_Y - 1) / TT_Y, ntz = (C.ax[2].n + TT_Z - 1) / TT_Z;
+  const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int q[3] = {(t / (ntz * nty)) * TT_X + (int)(threadIdx.x / (TT_Y * TT_Z)),
+                    ((t / ntz) % nty) * TT_Y + (int)((threadIdx.x / TT_Z) % TT_Y),
+                    (t % ntz) * TT_Z + (int)(threadIdx.x % TT_Z)};
+  if (q[0] >= C.ax[0].n || q[1] >= C.ax[1].n || q[2] >= C.ax[2].n) return;
   const int64_t n = C.ncell;
-  int q[3];
-  point_of(C, i, q);
+  const int64_t i = (int64_t)q[0] * C.ax[0].stride + (int64_t)q[1] * C.ax[1].stride + q[2];
   double acc[NCOMP];
   for (int c = 0; c < NCOMP; ++c) acc[c] = fbar[(F_VAL + c) * n + i];
   // first, pure second, upwind and Kreiss-Oliger rows along each axis
@@ -230,12 +243,13 @@ int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *stat
   if (!state || !state_bar || !work) return fail(BSSN_ERR_INVALID, "null buffer");
   if (!rhs_bar && !h_bar) return fail(BSSN_ERR_INVALID, "rhs_bar and h_bar are both null");
   const hipStream_t st = (hipStream_t)stream;
-  const int64_t b64 = (C.ncell + 63) / 64, b128 = (C.ncell + 127) / 128;
+  const int64_t b64 = (C.ncell + 63) / 64;
   double *feat = work, *fbar = work + (int64_t)NFEAT * C.ncell;
   if ((rc = launch_record_f64(C, state, feat, st))) return rc;
   hipLaunchKernelGGL(k_vjp_point, dim3((unsigned)b64), dim3(64), 0, st, C, feat, rhs_bar, h_bar, fbar);
   if ((rc = check_hip(hipGetLastError(), "adjoint point kernel launch"))) return rc;
-  hipLaunchKernelGGL(k_vjp_transpose, dim3((unsigned)b128), dim3(128), 0, st, C, state, fbar, state_bar, accumulate);
+  hipLaunchKernelGGL(k_vjp_transpose, dim3((unsigned)transpose_tiles(C)), dim3(TT_THREADS), 0, st, C, state, fbar,
+                     state_bar, accumulate);
   return check_hip(hipGetLastError(), "adjoint transpose kernel launch");
 }
 

@@ -286,7 +286,7 @@ def _momentum_constraint_and_derivative(v, inv_gamma, D):
     d2A = D.d2('A', A)
     d2W = D.d2('W', W)
     d2K = D.d2('K', K)
-    dM = np.zeros((3, 3) + K.shape)
+    dM = np.zeros((3, 3) + K.shape, dtype=K.dtype)
     for i in range(3):
         for l in range(3):
             div_d = np.zeros_like(K)
@@ -414,7 +414,7 @@ def _rhs_Gamma(v, params, inv_gamma, D, matter=None):
     # reference d2_shift[i, m, n] = diff1(d_shift[i, n], m): inner n, outer m
     d2_shift = np.einsum('nmi...->imn...', ddb)
     eighth = np.einsum('mn...,imn...->i...', inv_gamma, d2_shift)
-    div_deriv = np.zeros((3,) + v.trace_K.shape)
+    div_deriv = np.zeros((3,) + v.trace_K.shape, dtype=v.trace_K.dtype)
     for m in range(3):
         for n in range(3):
             # m == n: D2; else diff1(d_shift[n, n], m): inner n, outer m
@@ -530,7 +530,9 @@ def apply_sommerfeld_boundaries(v, rhs, params):
 # --------------------------------------------------------------------------
 
 def as_float64(v):
-    return BSSNVariables(*(np.asarray(f, dtype=np.float64) for f in v))
+    """FP64 fields; complex fields stay complex128, so complex-step
+    differentiation (u + i h du) runs through the same code."""
+    return BSSNVariables(*(np.asarray(f, dtype=np.result_type(f, np.float64)) for f in v))
 
 
 def compute_bssn_rhs(v, params):

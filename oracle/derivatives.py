@@ -39,7 +39,7 @@ def diff1_field(field, direction, dx, left_bc=0, right_bc=0, mad_q=1.0,
     axis; an active face replaces its outer two points by the one-sided
     closures of derivatives.py:99-153.
     """
-    field = np.asarray(field, dtype=np.float64)
+    field = np.asarray(field, dtype=np.result_type(field, np.float64))
     end = _end(field, direction, interior_end)
     f1, f2, f3 = (_shift(field, k, direction) for k in (1, 2, 3))
     b1, b2, b3 = (_shift(field, -k, direction) for k in (1, 2, 3))
@@ -84,7 +84,7 @@ def diff1_upwind_field(field, rhs_coefficient, direction, dx, left_bc=0,
     D4 is forced and the outer three points of each active face take the
     boundary-aware centred D4.
     """
-    field = np.asarray(field, dtype=np.float64)
+    field = np.asarray(field, dtype=np.result_type(field, np.float64))
     end = _end(field, direction, interior_end)
     f1, f2, f3 = (_shift(field, k, direction) for k in (1, 2, 3))
     b1, b2, b3 = (_shift(field, -k, direction) for k in (1, 2, 3))
@@ -106,7 +106,8 @@ def diff1_upwind_field(field, rhs_coefficient, direction, dx, left_bc=0,
         forward = mad_q * d4_forward + (1.0 - mad_q) * d6_forward
         backward = mad_q * d4_backward + (1.0 - mad_q) * d6_backward
         centered = mad_q * d4_centered + (1.0 - mad_q) * d6_centered
-    coeff = np.asarray(rhs_coefficient, dtype=np.float64)
+    # the side is chosen by the real part (piecewise: its derivative is zero)
+    coeff = np.real(np.asarray(rhs_coefficient)).astype(np.float64)
     derivative = np.where(coeff > 0.0, forward,
                           np.where(coeff < 0.0, backward, centered))
     derivative = np.array(np.broadcast_to(derivative, field.shape))
@@ -128,7 +129,7 @@ def diff1_upwind_field(field, rhs_coefficient, direction, dx, left_bc=0,
 def diff2_field(field, direction, dx, left_bc=0, right_bc=0, mad_q=1.0,
                 interior_start=0, interior_end=None):
     """Pure second derivative (reference derivatives.py:341-461)."""
-    field = np.asarray(field, dtype=np.float64)
+    field = np.asarray(field, dtype=np.result_type(field, np.float64))
     end = _end(field, direction, interior_end)
     f1, f2, f3 = (_shift(field, k, direction) for k in (1, 2, 3))
     b1, b2, b3 = (_shift(field, -k, direction) for k in (1, 2, 3))
@@ -177,7 +178,7 @@ D6_LEFT_CLOSURES = (
 def diff6_field(field, direction, dx, left_bc=0, right_bc=0, interior_start=0,
                 interior_end=None):
     """Sixth difference for Kreiss-Oliger (reference derivatives.py:463-599)."""
-    field = np.asarray(field, dtype=np.float64)
+    field = np.asarray(field, dtype=np.result_type(field, np.float64))
     end = _end(field, direction, interior_end)
     f1, f2, f3 = (_shift(field, k, direction) for k in (1, 2, 3))
     b1, b2, b3 = (_shift(field, -k, direction) for k in (1, 2, 3))

@@ -327,7 +327,7 @@ def _vjp(L, s, p, rbar=None, hbar=None):
     return out
 
 
-def _dot_test(name, lhs, rhs_, tol=1e-11):
+def _dot_test(name, lhs, rhs_, tol=1e-12):
     err = abs(lhs - rhs_) / max(abs(lhs), abs(rhs_))
     print(f"{name}: <w, J v> = {lhs:.15e}, <J^T w, v> = {rhs_:.15e}, rel diff {err:.1e} (bar {tol:.0e})", flush=True)
     if not err <= tol:
@@ -389,69 +389,108 @@ def case_adjoint_projection(L):
     _dot_test("projection (central differences)", lhs, float(np.sum(wb * dv)), tol=1e-8)
 
 
+def _step_vjp(L, s, p, w):
+    """J_step(s)^T w for one RK4 step, composed as jax_bssn_amd/adjoint.py
+    composes it: recomputed stages, transposed RHS and projections in
+    reverse order."""
+    cg = _lib.c_grid(s.shape[1:])
+    dt = p.dt
+
+    def F(u):
+        return rhs(L, unpack(u), p)
+
+    def P(u):
+        u = np.ascontiguousarray(u.copy())
+        check(L, L.bssn_enforce_constraints(ctypes.byref(cg), ptr(u), ptr(u), None))
+        return u
+
+    def FT(u, kb):
+        return _vjp(L, np.ascontiguousarray(u), p, rbar=np.ascontiguousarray(kb))
+
+    def PT(u, ub):
+        ub = np.ascontiguousarray(ub.copy())
+        check(L, L.bssn_project_vjp(ctypes.byref(cg), ptr(np.ascontiguousarray(u)), ptr(ub), None))
+        return ub
+
+    u0 = P(s)
+    k1 = F(u0)
+    pre1 = u0 + 0.5 * dt * k1
+    m1 = P(pre1)
+    k2 = F(m1)
+    pre2 = u0 + 0.5 * dt * k2
+    m2 = P(pre2)
+    k3 = F(m2)
+    pre3 = u0 + dt * k3
+    m3 = P(pre3)
+    k4 = F(m3)
+    pre_out = u0 + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    pb = PT(pre_out, w)
+    u0b = pb.copy()
+    k1b, k2b, k3b, k4b = dt / 6.0 * pb, dt / 3.0 * pb, dt / 3.0 * pb, dt / 6.0 * pb
+    b = PT(pre3, FT(m3, k4b))
+    u0b += b
+    k3b += dt * b
+    b = PT(pre2, FT(m2, k3b))
+    u0b += b
+    k2b += 0.5 * dt * b
+    b = PT(pre1, FT(m1, k2b))
+    u0b += b
+    k1b += 0.5 * dt * b
+    u0b += FT(u0, k1b)
+    return PT(s, u0b)
+
+
 def case_adjoint_step(L):
-    """The reverse sweep of one RK4 step composed as jax_bssn_amd/adjoint.py
-    composes it (recomputed stages, transposed RHS and projections, in
-    reverse order) against the tangent of the fused evolution step:
-    <w, J_step v> = <J_step^T w, v>, Sommerfeld and periodic grids."""
+    """The reverse sweep of one RK4 step against the tangent of the fused
+    evolution step: <w, J_step v> = <J_step^T w, v>, Sommerfeld and periodic
+    grids."""
     rng = np.random.default_rng(9)
     for name, (v, p) in _adjoint_configs().items():
         if p.mad_q != 1.0:
             continue
         s = pack(v)
-        shape = s.shape[1:]
-        cp, cg = _lib.c_params(p), _lib.c_grid(shape)
-        dt = p.dt
-
-        def F(u):
-            return rhs(L, unpack(u), p)
-
-        def P(u):
-            u = np.ascontiguousarray(u.copy())
-            check(L, L.bssn_enforce_constraints(ctypes.byref(cg), ptr(u), ptr(u), None))
-            return u
-
-        def FT(u, kb):
-            return _vjp(L, np.ascontiguousarray(u), p, rbar=np.ascontiguousarray(kb))
-
-        def PT(u, ub):
-            ub = np.ascontiguousarray(ub.copy())
-            check(L, L.bssn_project_vjp(ctypes.byref(cg), ptr(np.ascontiguousarray(u)), ptr(ub), None))
-            return ub
-
+        cp, cg = _lib.c_params(p), _lib.c_grid(s.shape[1:])
         dv = np.ascontiguousarray(1e-2 * rng.standard_normal(s.shape))
         w = rng.standard_normal(s.shape)
         u, du = s.copy(), dv.copy()
         work = np.zeros(L.bssn_tangent_work_size(ctypes.byref(cg), 1))
         check(L, L.bssn_evolve_tangent(ctypes.byref(cp), ctypes.byref(cg), 1, 1, ptr(u), ptr(du), ptr(work), None))
-        lhs = float(np.sum(w * du))
-        u0 = P(s)
-        k1 = F(u0)
-        pre1 = u0 + 0.5 * dt * k1
-        m1 = P(pre1)
-        k2 = F(m1)
-        pre2 = u0 + 0.5 * dt * k2
-        m2 = P(pre2)
-        k3 = F(m2)
-        pre3 = u0 + dt * k3
-        m3 = P(pre3)
-        k4 = F(m3)
-        pre_out = u0 + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
-        pb = PT(pre_out, w)
-        u0b = pb.copy()
-        k1b, k2b, k3b, k4b = dt / 6 * pb, dt / 3 * pb, dt / 3 * pb, dt / 6 * pb
-        b = PT(pre3, FT(m3, k4b))
-        u0b += b
-        k3b += dt * b
-        b = PT(pre2, FT(m2, k3b))
-        u0b += b
-        k2b += 0.5 * dt * b
-        b = PT(pre1, FT(m1, k2b))
-        u0b += b
-        k1b += 0.5 * dt * b
-        u0b += FT(u0, k1b)
-        ub = PT(s, u0b)
-        _dot_test(f"RK4 step, {name}", lhs, float(np.sum(ub * dv)), tol=1e-10)
+        _dot_test(f"RK4 step, {name}", float(np.sum(w * du)), float(np.sum(_step_vjp(L, s, p, w) * dv)))
+
+
+def _complex_step(fn, v, dv, h=1e-30):
+    """Directional derivative of the oracle map fn at v along dv, exact to
+    rounding: Im fn(v + i h dv) / h (the oracle's arithmetic is analytic
+    away from the upwind side select, whose derivative is zero)."""
+    out = fn(OB.BSSNVariables(*[np.asarray(a, dtype=complex) + 1j * h * d for a, d in zip(v, dv)]))
+    return np.imag(out) / h if isinstance(out, np.ndarray) else OB.BSSNVariables(*[np.imag(x) / h for x in out])
+
+
+def case_adjoint_oracle(L):
+    """The adjoint kernels against the derivative of the FP64 oracle itself
+    (complex step through oracle/bssn.py, the stand-in for jax.grad of the
+    reference): <w, J_oracle v> = <J^T w, v> for the RHS, the Hamiltonian
+    constraint and one RK4 step.  The two sides differentiate the same
+    discrete maps evaluated in different orders (observed <= 2e-15): 1e-12."""
+    rng = np.random.default_rng(10)
+    configs = _adjoint_configs()
+    for name in ("periodic 1+log", "Sommerfeld all faces", "MAD"):
+        v, p = configs[name]
+        s = pack(v)
+        dv = OB.BSSNVariables(*[1e-2 * rng.standard_normal(np.shape(x)) for x in v])
+        sym = lambda t: 0.5 * (t + np.swapaxes(t, 0, 1))  # the packed state holds symmetric tensors
+        dv = dv._replace(conformal_metric=sym(dv.conformal_metric), traceless_K=sym(dv.traceless_K))
+        dvs = pack(dv)
+        w = np.ascontiguousarray(rng.standard_normal(s.shape))
+        jv = pack(_complex_step(lambda u: OB.compute_bssn_rhs(u, p), v, dv))
+        _dot_test(f"RHS vs oracle, {name}", float(np.sum(w * jv)), float(np.sum(_vjp(L, s, p, rbar=w) * dvs)))
+        hb = np.ascontiguousarray(rng.standard_normal(s.shape[1:]))
+        jh = _complex_step(lambda u: OB.compute_hamiltonian_constraint(u, p), v, dv)
+        _dot_test(f"H vs oracle, {name}", float(np.sum(hb * jh)), float(np.sum(_vjp(L, s, p, hbar=hb) * dvs)))
+        if p.mad_q == 1.0:
+            js = pack(_complex_step(lambda u: OB.rk4_step(u, p), v, dv))
+            _dot_test(f"RK4 step vs oracle, {name}", float(np.sum(w * js)),
+                      float(np.sum(_step_vjp(L, s, p, w) * dvs)))
 
 
 CASES = {name[5:]: fn for name, fn in globals().items() if name.startswith("case_")}

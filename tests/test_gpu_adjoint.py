@@ -4,11 +4,13 @@ bssn_rhs_vjp / bssn_project_vjp), through the C ABI.
 The adjoint of a linear map is pinned by the dot-product identity
 <w, J v> = <J^T w, v> for random v, w, with J v from the forward-mode
 tangent kernels (an independent implementation: dual numbers through the
-evolution's own stage kernels).  Bars: 1e-10 relative for one RHS, H or
-step (both sides are exact derivatives; only rounding separates them), the
-same over several checkpointed steps, and 1e-8 for the gradient of the L2
-Hamiltonian constraint against the forward-mode gradient (the task's
-bar against jax.grad)."""
+evolution's own stage kernels) and from the FP64 oracle itself (complex
+step through oracle/bssn.py, the stand-in for jax.grad of the reference).
+Bars: 1e-12 relative for one RHS, H or step (both sides are exact
+derivatives; only rounding separates them), the same over several
+checkpointed steps, and 1e-8 for the gradient of the L2 Hamiltonian
+constraint against the forward-mode gradient (the task's bar against
+jax.grad)."""
 
 import numpy as np
 import pytest
@@ -77,7 +79,7 @@ def test_rhs_and_hamiltonian_adjoints_match_tangents(J, case):
     w = torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=g)
     ops = A._Ops(s, p)
     _, drhs = T.rhs_tangent(s, dv, p)
-    _check(f"RHS {case}", float(torch.sum(w * drhs)), float(torch.sum(ops.FT(s, w) * dv)), 1e-10)
+    _check(f"RHS {case}", float(torch.sum(w * drhs)), float(torch.sum(ops.FT(s, w) * dv)), 1e-12)
     hb = torch.randn(s.shape[1:], dtype=torch.float64, device="cuda", generator=g)
     H = torch.empty(s.shape[1:], dtype=torch.float64, device="cuda")
     dH = torch.empty_like(H)
@@ -87,7 +89,7 @@ def test_rhs_and_hamiltonian_adjoints_match_tangents(J, case):
     _lib.check(_lib.load().bssn_hamiltonian_tangent(ctypes.byref(_lib.c_params(p)),
                                                     ctypes.byref(_lib.c_grid(s.shape[1:])), 1, _ptr(s), _ptr(dv),
                                                     _ptr(H), _ptr(dH), stream_handle()))
-    _check(f"H {case}", float(torch.sum(hb * dH)), float(torch.sum(ops.HT(s, hb.contiguous()) * dv)), 1e-10)
+    _check(f"H {case}", float(torch.sum(hb * dH)), float(torch.sum(ops.HT(s, hb.contiguous()) * dv)), 1e-12)
 
 
 @pytest.mark.parametrize("case", ["sommerfeld_tiles", "periodic"])
@@ -105,10 +107,39 @@ def test_checkpointed_steps_adjoint_matches_tangent(J, case):
     w = torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=g)
     _, du = T.evolve_tangent(s, dv, p, 6)
     ub, _ = A.evolve_vjp(s, 6, p, w, checkpoint_every=4)
-    _check(f"6 steps {case}", float(torch.sum(w * du)), float(torch.sum(ub * dv)), 1e-10)
+    _check(f"6 steps {case}", float(torch.sum(w * du)), float(torch.sum(ub * dv)), 1e-12)
     # one step without checkpoints equals the single-step adjoint
     _, du1 = T.evolve_tangent(s, dv, p, 1)
-    _check(f"1 step {case}", float(torch.sum(w * du1)), float(torch.sum(A.rk4_step_vjp(s, w, p) * dv)), 1e-10)
+    _check(f"1 step {case}", float(torch.sum(w * du1)), float(torch.sum(A.rk4_step_vjp(s, w, p) * dv)), 1e-12)
+
+
+def _complex_step(fn, v, dv, h=1e-30):
+    """Im fn(v + i h dv) / h: the oracle's exact directional derivative."""
+    out = fn(OB.BSSNVariables(*[np.asarray(a, dtype=complex) + 1j * h * d for a, d in zip(v, dv)]))
+    return OB.BSSNVariables(*[np.imag(x) / h for x in out])
+
+
+@pytest.mark.parametrize("case", ["sommerfeld_tiles", "periodic"])
+def test_adjoint_matches_the_oracle_derivative(J, case):
+    """<w, J_oracle v> with J_oracle v the complex-step derivative of the
+    oracle's RHS and RK4 step against <J^T w, v> from the GPU adjoint."""
+    from jax_bssn_amd import adjoint as A
+    v, p = CASES[case]()
+    print(f"[adjoint vs oracle] case={case}", flush=True)
+    rng = np.random.default_rng(4)
+    sym = lambda t: 0.5 * (t + np.swapaxes(t, 0, 1))
+    dv = OB.BSSNVariables(*[1e-2 * rng.standard_normal(np.shape(x)) for x in v])
+    dv = dv._replace(conformal_metric=sym(dv.conformal_metric), traceless_K=sym(dv.traceless_K))
+    jv = _dev(J, _complex_step(lambda u: OB.compute_bssn_rhs(u, p), v, dv))
+    js = _dev(J, _complex_step(lambda u: OB.rk4_step(u, p), v, dv))
+    pj = _params(J, p)
+    s, dvs = _dev(J, v), _dev(J, dv)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    w = torch.randn(s.shape, dtype=torch.float64, device="cuda", generator=g)
+    ops = A._Ops(s, pj)
+    _check(f"RHS vs oracle {case}", float(torch.sum(w * jv)), float(torch.sum(ops.FT(s, w) * dvs)), 1e-12)
+    _check(f"step vs oracle {case}", float(torch.sum(w * js)),
+           float(torch.sum(A.rk4_step_vjp(s, w, pj, ops) * dvs)), 1e-12)
 
 
 def test_reverse_gradient_matches_forward_mode_on_a_puncture(J):

@@ -334,3 +334,31 @@ def test_matter_sources_enter_with_reference_coefficients():
     c = B.compute_all_constraints_with_matter(v, p, rho, S)
     np.testing.assert_allclose(c.hamiltonian, -16 * np.pi * rho, atol=1e-13)
     np.testing.assert_allclose(c.momentum, -8 * np.pi * S, atol=1e-13)
+
+
+def test_complex_step_through_the_oracle_is_its_derivative():
+    """The oracle runs on complex fields (the upwind side chosen by the real
+    part of the shift), so Im F(u + i h du) / h is the exact derivative of
+    its RHS and of its RK4 step: it must match real central differences of
+    the same maps to their truncation error.  The adjoint tests use this
+    derivative as the stand-in for jax.grad of the reference."""
+    import warnings
+    from tests.parity_utils import smooth_state
+    v = smooth_state((10, 12, 14), seed=5)
+    p = B.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=0.1, dt=0.025, zero_shift=0, gauge=1, xl_bc=1, xr_bc=1,
+                         zl_bc=1, x_min=-0.45, y_min=-0.55, z_min=-0.65)
+    rng = np.random.default_rng(3)
+    sym = lambda t: 0.5 * (t + np.swapaxes(t, 0, 1))
+    dv = B.BSSNVariables(*[1e-2 * rng.standard_normal(np.shape(x)) for x in v])
+    dv = dv._replace(conformal_metric=sym(dv.conformal_metric), traceless_K=sym(dv.traceless_K))
+    h, e = 1e-30, 1e-6
+    for fn in (lambda u: B.compute_bssn_rhs(u, p), lambda u: B.rk4_step(u, p)):
+        with warnings.catch_warnings():
+            warnings.simplefilter("error", np.exceptions.ComplexWarning)  # no silent cast to real
+            cs = fn(B.BSSNVariables(*[np.asarray(a, dtype=complex) + 1j * h * d for a, d in zip(v, dv)]))
+        plus = fn(B.BSSNVariables(*[a + e * d for a, d in zip(v, dv)]))
+        minus = fn(B.BSSNVariables(*[a - e * d for a, d in zip(v, dv)]))
+        for c, a, b in zip(cs, plus, minus):
+            fd = (a - b) / (2 * e)
+            assert np.abs(np.imag(c) / h - fd).max() <= 1e-7 * np.abs(fd).max()
+            assert np.abs(np.real(c) - 0.5 * (a + b)).max() <= 1e-9 * max(np.abs(a).max(), 1.0)
