@@ -308,8 +308,9 @@ int bssn_hamiltonian_tangent(const bssn_params *p, const bssn_grid *grid, int nd
  * + J_H(state)^T h_bar, with F the RHS of compute_bssn_rhs (time_evolve.py:
  * 41-57) and H the Hamiltonian constraint (bssn/constraints.py:271-303);
  * rhs_bar (21 * n) or h_bar (n) may be null.  work holds
- * bssn_vjp_work_size(grid) doubles (the features of every point and their
- * adjoints).  The upwind side of each point is held
+ * bssn_vjp_work_size(grid) doubles (the features of every point, their
+ * adjoints and per-axis tables of the transposed stencil rows).  The upwind
+ * side of each point is held
  * fixed, as jax differentiates jnp.where. */
 int64_t bssn_vjp_work_size(const bssn_grid *grid);
 int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *state, const double *rhs_bar,

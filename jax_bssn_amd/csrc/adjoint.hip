@@ -107,6 +107,44 @@ __device__ __forceinline__ int reach(const Axis &A, int qa) {
   return r < A.n ? r : A.n - 1;
 }
 
+// Transposed row coefficients along each axis, for every position q and
+// tap offset d in [-7, 7]: the coefficient at offset d of the rows of the
+// point p = q - d -- first and second derivative, sixth difference times
+// the dissipation factor, and the upwind row for a positive, negative and
+// zero shift at p.  Rows depend on the position along one axis only, so the
+// points near closures read them instead of building closure rows per tap.
+// Layout: axis 0 positions, then axis 1, then axis 2, each (q, d + 7, kind).
+constexpr int TR_D = 15, TR_K = 6;
+
+__host__ __device__ inline int64_t table_doubles(int nx, int ny, int nz) {
+  return (int64_t)(nx + ny + nz) * TR_D * TR_K;
+}
+
+__device__ __forceinline__ int table_row(const Ctx &C, int a, int qa) {
+  return (a > 0 ? C.ax[0].n : 0) + (a > 1 ? C.ax[1].n : 0) + qa;
+}
+
+__global__ void __launch_bounds__(256) k_vjp_tables(Ctx C, double *__restrict__ tab) {
+  const int id = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (id >= (C.ax[0].n + C.ax[1].n + C.ax[2].n) * TR_D) return;
+  int qa = id / TR_D, a = 0;
+  while (a < 2 && qa >= C.ax[a].n) qa -= C.ax[a++].n;
+  const Axis &A = C.ax[a];
+  const int d = id % TR_D - 7;
+  double *T = tab + (int64_t)id * TR_K;
+  if (A.reduced) {
+    for (int k = 0; k < TR_K; ++k) T[k] = 0.0;
+    return;
+  }
+  const int p = wrap_index(qa - d, A.n);
+  T[0] = tap(row_d1<double>(A, p, C.inv_dx, false), d);
+  T[1] = tap(row_d2<double>(A, p, C.inv_dx2), d);
+  T[2] = C.ko * tap(row_d6<double>(A, p), d);
+  T[3] = tap(row_upwind<double>(A, p, C.inv_dx, 1.0), d);
+  T[4] = tap(row_upwind<double>(A, p, C.inv_dx, -1.0), d);
+  T[5] = tap(row_upwind<double>(A, p, C.inv_dx, 0.0), d);
+}
+
 // Points of the transpose in 4 x 4 x 16 tiles (x, y, z; z fastest over
 // the lanes), one tile per 256-thread workgroup, tiles in x-major order
 // spread over the XCDs in contiguous ranges: the taps of neighbouring rows
@@ -119,6 +157,7 @@ __host__ __device__ inline int64_t transpose_tiles(const Ctx &C) {
 
 __global__ void __launch_bounds__(TT_THREADS) k_vjp_transpose(Ctx C, const double *__restrict__ us,
                                                               const double *__restrict__ fbar,
+                                                              const double *__restrict__ tab,
                                                               double *__restrict__ ubar, int accumulate) {
   const int nty = (C.ax[1].n + TT_Y - 1) / TT_Y, ntz = (C.ax[2].n + TT_Z - 1) / TT_Z;
   const int t = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -150,10 +189,11 @@ __global__ void __launch_bounds__(TT_THREADS) k_vjp_transpose(Ctx C, const doubl
         c6 = C.ko * D6_TAPS[k];
         cu = (bp > 0.0 ? UPF_TAPS[k] : (bp < 0.0 ? UPB_TAPS[k] : D1_TAPS[k])) * C.inv_dx;
       } else {
-        c1 = tap(row_d1<double>(A, p[a], C.inv_dx, false), d);
-        c2 = tap(row_d2<double>(A, p[a], C.inv_dx2), d);
-        c6 = C.ko * tap(row_d6<double>(A, p[a]), d);
-        cu = tap(row_upwind<double>(A, p[a], C.inv_dx, bp), d);
+        const double *T = tab + ((int64_t)table_row(C, a, q[a]) * TR_D + d + 7) * TR_K;
+        c1 = T[0];
+        c2 = T[1];
+        c6 = T[2];
+        cu = bp > 0.0 ? T[3] : (bp < 0.0 ? T[4] : T[5]);
       }
       if (c1 == 0.0 && c2 == 0.0 && c6 == 0.0 && cu == 0.0) continue;
       for (int c = 0; c < NCOMP; ++c) {
@@ -174,12 +214,13 @@ __global__ void __launch_bounds__(TT_THREADS) k_vjp_transpose(Ctx C, const doubl
       const bool plain_a = Ra == 3 && A.q == 1.0, plain_b = Rb == 3 && B.q == 1.0;
       for (int da = -Ra; da <= Ra; ++da) {
         const int pa = wrap_index(q[a] - da, A.n);
-        const double ca = plain_a ? D1_TAPS[da + 3] * C.inv_dx : tap(row_d1<double>(A, pa, C.inv_dx, false), da);
+        const double ca = plain_a ? D1_TAPS[da + 3] * C.inv_dx
+                                  : tab[((int64_t)table_row(C, a, q[a]) * TR_D + da + 7) * TR_K];
         if (ca == 0.0) continue;
         for (int db = -Rb; db <= Rb; ++db) {
           const int pb = wrap_index(q[bx] - db, B.n);
-          const double cb =
-              plain_b ? D1_TAPS[db + 3] * C.inv_dx : tap(row_d1<double>(B, pb, C.inv_dx, false), db);
+          const double cb = plain_b ? D1_TAPS[db + 3] * C.inv_dx
+                                    : tab[((int64_t)table_row(C, bx, q[bx]) * TR_D + db + 7) * TR_K];
           if (cb == 0.0) continue;
           int p[3] = {q[0], q[1], q[2]};
           p[a] = pa;
@@ -232,7 +273,8 @@ extern "C" {
 
 int64_t bssn_vjp_work_size(const bssn_grid *grid) {
   if (!grid || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail(BSSN_ERR_INVALID, "invalid grid");
-  return 2 * (int64_t)NFEAT * grid->nx * grid->ny * grid->nz;  // features and their adjoints
+  // features, their adjoints and the transposed row tables
+  return 2 * (int64_t)NFEAT * grid->nx * grid->ny * grid->nz + table_doubles(grid->nx, grid->ny, grid->nz);
 }
 
 int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *state, const double *rhs_bar,
@@ -248,7 +290,11 @@ int bssn_rhs_vjp(const bssn_params *p, const bssn_grid *grid, const double *stat
   if ((rc = launch_record_f64(C, state, feat, st))) return rc;
   hipLaunchKernelGGL(k_vjp_point, dim3((unsigned)b64), dim3(64), 0, st, C, feat, rhs_bar, h_bar, fbar);
   if ((rc = check_hip(hipGetLastError(), "adjoint point kernel launch"))) return rc;
-  hipLaunchKernelGGL(k_vjp_transpose, dim3((unsigned)transpose_tiles(C)), dim3(TT_THREADS), 0, st, C, state, fbar,
+  double *tab = fbar + (int64_t)NFEAT * C.ncell;
+  const int nrow = (C.ax[0].n + C.ax[1].n + C.ax[2].n) * TR_D;
+  hipLaunchKernelGGL(k_vjp_tables, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, st, C, tab);
+  if ((rc = check_hip(hipGetLastError(), "adjoint table kernel launch"))) return rc;
+  hipLaunchKernelGGL(k_vjp_transpose, dim3((unsigned)transpose_tiles(C)), dim3(TT_THREADS), 0, st, C, state, fbar, tab,
                      state_bar, accumulate);
   return check_hip(hipGetLastError(), "adjoint transpose kernel launch");
 }

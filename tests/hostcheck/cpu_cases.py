@@ -343,6 +343,8 @@ def _adjoint_configs():
                                                             zero_shift=1, gauge=0)),
         "Sommerfeld all faces": (smooth_state((16, 17, 18), seed=63), sommerfeld_params()),
         "Sommerfeld and periodic": (smooth_state((18, 16, 14), seed=64), sommerfeld_params(yl_bc=0, yr_bc=0)),
+        # a plain interior [10, n - 10) on every axis: LDS tiles (partial ones included) and the gather shell
+        "Sommerfeld, plain interior": (smooth_state((28, 26, 30), seed=66), sommerfeld_params()),
         "MAD": (smooth_state((14, 14, 14), seed=65),
                 OB.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=1.0 / 14, dt=0.02, zero_shift=0, gauge=1,
                                   mad_q=0.25)),
@@ -474,7 +476,7 @@ def case_adjoint_oracle(L):
     discrete maps evaluated in different orders (observed <= 2e-15): 1e-12."""
     rng = np.random.default_rng(10)
     configs = _adjoint_configs()
-    for name in ("periodic 1+log", "Sommerfeld all faces", "MAD"):
+    for name in ("periodic 1+log", "Sommerfeld all faces", "MAD", "Sommerfeld, plain interior"):
         v, p = configs[name]
         s = pack(v)
         dv = OB.BSSNVariables(*[1e-2 * rng.standard_normal(np.shape(x)) for x in v])

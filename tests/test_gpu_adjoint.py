@@ -53,6 +53,8 @@ CASES = {
     "sommerfeld_tiles": lambda: _sommerfeld((24, 24, 24), 71),
     "sommerfeld_points": lambda: _sommerfeld((14, 16, 18), 72),
     "periodic": lambda: _periodic((16, 16, 16), 73),
+    # plain interior [10, n - 10): LDS tiles of the transpose, partial tiles and the gather shell
+    "sommerfeld_interior": lambda: _sommerfeld((40, 36, 44), 74),
 }
 
 
@@ -119,7 +121,7 @@ def _complex_step(fn, v, dv, h=1e-30):
     return OB.BSSNVariables(*[np.imag(x) / h for x in out])
 
 
-@pytest.mark.parametrize("case", ["sommerfeld_tiles", "periodic"])
+@pytest.mark.parametrize("case", ["sommerfeld_tiles", "periodic", "sommerfeld_interior"])
 def test_adjoint_matches_the_oracle_derivative(J, case):
     """<w, J_oracle v> with J_oracle v the complex-step derivative of the
     oracle's RHS and RK4 step against <J^T w, v> from the GPU adjoint."""
