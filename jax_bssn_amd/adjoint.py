@@ -85,35 +85,41 @@ def rk4_step_vjp(state, state_bar, params, ops=None):
     device states (state: the input of the step)."""
     ops = ops or _Ops(state, params)
     dt = float(params.dt)
+    # (each line one fused elementwise kernel over the state)
     u0 = ops.P(state)
     k1 = ops.F(u0)
-    pre1 = u0 + 0.5 * dt * k1
+    pre1 = torch.add(u0, k1, alpha=0.5 * dt)
     m1 = ops.P(pre1)
     k2 = ops.F(m1)
-    pre2 = u0 + 0.5 * dt * k2
+    pre2 = torch.add(u0, k2, alpha=0.5 * dt)
     m2 = ops.P(pre2)
     k3 = ops.F(m2)
-    pre3 = u0 + dt * k3
+    pre3 = torch.add(u0, k3, alpha=dt)
     m3 = ops.P(pre3)
     k4 = ops.F(m3)
-    pre_out = u0 + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
-    del k1, k2, k3, k4
+    acc = torch.add(k1, k2, alpha=2.0)
+    del k1, k2
+    acc.add_(k3, alpha=2.0).add_(k4)
+    del k3, k4
+    pre_out = torch.add(u0, acc, alpha=dt / 6.0)
+    del acc
     pbar = ops.PT(pre_out, state_bar.clone())
     del pre_out
     u0_bar = pbar.clone()
-    k1_bar, k2_bar, k3_bar, k4_bar = (dt / 6.0) * pbar, (dt / 3.0) * pbar, (dt / 3.0) * pbar, (dt / 6.0) * pbar
+    k1_bar, k4_bar = torch.mul(pbar, dt / 6.0), torch.mul(pbar, dt / 6.0)
+    k2_bar, k3_bar = torch.mul(pbar, dt / 3.0), torch.mul(pbar, dt / 3.0)
     # stage 4 <- m3 <- pre3 = u0 + dt k3
     b = ops.PT(pre3, ops.FT(m3, k4_bar))
     u0_bar += b
-    k3_bar += dt * b
+    k3_bar.add_(b, alpha=dt)
     # stage 3 <- m2 <- pre2 = u0 + dt/2 k2
     b = ops.PT(pre2, ops.FT(m2, k3_bar))
     u0_bar += b
-    k2_bar += 0.5 * dt * b
+    k2_bar.add_(b, alpha=0.5 * dt)
     # stage 2 <- m1 <- pre1 = u0 + dt/2 k1
     b = ops.PT(pre1, ops.FT(m1, k2_bar))
     u0_bar += b
-    k1_bar += 0.5 * dt * b
+    k1_bar.add_(b, alpha=0.5 * dt)
     # stage 1 <- u0
     ops.FT(u0, k1_bar, out=u0_bar, accumulate=True)
     return ops.PT(state, u0_bar)

@@ -42,10 +42,16 @@ struct StridedFeatures {
   int64_t n;
   __device__ __forceinline__ double operator()(int k) const { return p[k * n]; }
 };
+// The upwind and dissipation features enter the RHS linearly, with
+// adjoints rbar_c b_a and rbar_c: they are not stored; the point kernel
+// stores the effective rbar_c in the dissipation slots instead and the
+// transpose applies the upwind rows times the shift at their point.
 struct StridedAdjoints {
   double *p;
   int64_t n;
-  __device__ __forceinline__ void put(int k, double v) { p[k * n] = v; }
+  __device__ __forceinline__ void put(int k, double v) {
+    if (k < F_UP) p[k * n] = v;
+  }
 };
 struct StridedAdd {
   double *p;
@@ -73,6 +79,8 @@ __global__ void __launch_bounds__(64) k_vjp_point(Ctx C, const double *feat, con
     for (int c = 0; c < NCOMP; ++c) rb[c] = face ? 0.0 : rbar[c * n + i];
   StridedAdjoints out{fbar + i, n};
   rhs_vjp_point(C, StridedFeatures{feat + i, n}, rbar ? rb : (const double *)nullptr, hbar ? hbar[i] : 0.0, out);
+  for (int c = 0; c < NCOMP; ++c)
+    fbar[(F_KO + c) * n + i] = (rbar && !(c >= BETA && C.zero_shift)) ? rb[c] : 0.0;
   if (face) {
     double xr[3], rf[NCOMP];
     for (int a = 0; a < 3; ++a) xr[a] = C.xmin[a] + C.dx * (double)q[a];
@@ -196,13 +204,17 @@ __global__ void __launch_bounds__(TT_THREADS) k_vjp_transpose(Ctx C, const doubl
         cu = bp > 0.0 ? T[3] : (bp < 0.0 ? T[4] : T[5]);
       }
       if (c1 == 0.0 && c2 == 0.0 && c6 == 0.0 && cu == 0.0) continue;
-      for (int c = 0; c < NCOMP; ++c) {
-        double v = c1 * fbar[(F_D1 + 3 * c + a) * n + j] + cu * fbar[(F_UP + 3 * c + a) * n + j] +
-                   c6 * fbar[(F_KO + c) * n + j];
-        const int s = d2_slot(c);
-        if (s >= 0) v += c2 * fbar[(F_D2 + 6 * s + sym(a, a)) * n + j];
-        acc[c] += v;
-      }
+      if (c1 != 0.0)
+        for (int c = 0; c < NCOMP; ++c) acc[c] += c1 * fbar[(F_D1 + 3 * c + a) * n + j];
+      // upwind (times the shift at p) and dissipation rows on the effective rbar at p
+      const double cr = cu * bp + c6;
+      if (cr != 0.0)
+        for (int c = 0; c < NCOMP; ++c) acc[c] += cr * fbar[(F_KO + c) * n + j];
+      if (c2 != 0.0)
+        for (int c = 0; c < NCOMP; ++c) {
+          const int s = d2_slot(c);
+          if (s >= 0) acc[c] += c2 * fbar[(F_D2 + 6 * s + sym(a, a)) * n + j];
+        }
     }
   }
   // mixed second derivatives: products of the two axes' first-derivative rows
