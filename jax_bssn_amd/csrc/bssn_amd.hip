@@ -87,17 +87,16 @@ __global__ void __launch_bounds__(256) k_combine(int64_t n, StageOut O, const do
 
 // Reference bssn/constraints.py:22-97 (momentum), 271-303 (Hamiltonian),
 // 318-350 (det, trace), 353-384 (Gamma constraint).
-__global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__restrict__ us,
-                                                     const double *__restrict__ matter, double *out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.ncell) return;
-  const int z = (int)(i % C.ax[2].n);
-  const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
-  const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
-  PointProvider<double> pv;
-  pv.setup(C, us, x, y, z);
+// The constraints at the point (x, y, z) (index i) from the stencils of the
+// provider pv (global-memory rows of one point, or the LDS boxes of a tile,
+// where every thread of the tile stages every component and only threads
+// with inside set store).
+template <class PS>
+__device__ __forceinline__ void constraints_eval(const Ctx &C, PS &pv, const double *__restrict__ us,
+                                                 const double *__restrict__ matter, int x, int y, int z, int64_t i,
+                                                 double *out, bool inside) {
   double u[NCOMP];
-  load_point(C, us, pv.idx, u);
+  load_point(C, us, i, u);
   double g[6], A[6], ig[6];
   for (int c = 0; c < 6; ++c) {
     g[c] = u[GXX + c];
@@ -176,37 +175,42 @@ __global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__rest
     Aup[c] = acc;
   }
   const int64_t n = C.ncell;
-  out[0 * n + i] = contract(ig, R) + 2.0 / 3.0 * K * K - contract(A, Aup) -
-                   (matter ? 16.0 * M_PI * matter[i] : 0.0);
+  const double H = contract(ig, R) + 2.0 / 3.0 * K * K - contract(A, Aup) - (matter ? 16.0 * M_PI * matter[i] : 0.0);
 
-  // momentum constraint
+  // momentum constraint: first[r] = d_j B_r^j with B_r^j = gamma^{jk} A_rk
+  // taken as a grid function (the derivative rows applied to B at the
+  // taps); the inverse metric of each tap point is formed once for all r
   const double Wf = W > 1.0e-12 ? W : 1.0e-12;
-  auto b_at = [&](int a, int o, int r, int col) -> double {
-    // B_r^col = gamma^{col k} A_{r k} at the point offset by o along axis a
-    double gg[6], AA[3], ii[6];
-    int q[3] = {x, y, z};
-    q[a] = wrap_index(q[a] + o, C.ax[a].n);
-    const int64_t j = (int64_t)q[0] * C.ax[0].stride + (int64_t)q[1] * C.ax[1].stride + q[2];
-    for (int c = 0; c < 6; ++c) gg[c] = us[(GXX + c) * n + j];
-    for (int k = 0; k < 3; ++k) AA[k] = us[(AXX + sym(r, k)) * n + j];
-    inverse_metric(gg, ii);
-    return ii[sym(col, 0)] * AA[0] + ii[sym(col, 1)] * AA[1] + ii[sym(col, 2)] * AA[2];
-  };
+  double first[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < 3; ++j) {
+    if (C.ax[j].reduced) continue;
+    const Row8<double> row = row_d1<double>(C.ax[j], j == 0 ? x : (j == 1 ? y : z), C.inv_dx, false);
+    double sj[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 8; ++k) {
+      if (row.c[k] == 0.0) continue;
+      int q[3] = {x, y, z};
+      q[j] = wrap_index(q[j] + row.o0 + k, C.ax[j].n);
+      const int64_t t = (int64_t)q[0] * C.ax[0].stride + (int64_t)q[1] * C.ax[1].stride + q[2];
+      double gg[6], AA[6], ii[6];
+      for (int c = 0; c < 6; ++c) {
+        gg[c] = us[(GXX + c) * n + t];
+        AA[c] = us[(AXX + c) * n + t];
+      }
+      inverse_metric(gg, ii);
+      for (int r = 0; r < 3; ++r)
+        sj[r] += row.c[k] * (ii[sym(j, 0)] * AA[sym(r, 0)] + ii[sym(j, 1)] * AA[sym(r, 1)] +
+                             ii[sym(j, 2)] * AA[sym(r, 2)]);
+    }
+    for (int r = 0; r < 3; ++r) first[r] += sj[r];
+  }
   double dA[6][3];
   for (int c = 0; c < 6; ++c) {
     pv.stage_comp(AXX + c);
     for (int d = 0; d < 3; ++d) dA[c][d] = pv.d1(d);
   }
+  if (!inside) return;
+  out[0 * n + i] = H;
   for (int r = 0; r < 3; ++r) {
-    double first = 0.0;
-    for (int j = 0; j < 3; ++j) {
-      if (C.ax[j].reduced) continue;
-      const Row8<double> row = row_d1<double>(C.ax[j], j == 0 ? x : (j == 1 ? y : z), C.inv_dx, false);
-      double s = 0.0;
-      for (int k = 0; k < 8; ++k)
-        if (row.c[k] != 0.0) s += row.c[k] * b_at(j, row.o0 + k, r, j);
-      first += s;
-    }
     double Bw = 0.0;  // B_r^j d_j W
     for (int j = 0; j < 3; ++j) {
       double B = ig[sym(j, 0)] * A[sym(r, 0)] + ig[sym(j, 1)] * A[sym(r, 1)] + ig[sym(j, 2)] * A[sym(r, 2)];
@@ -214,7 +218,7 @@ __global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__rest
     }
     double second = 0.0;
     for (int c = 0; c < 6; ++c) second += (SYM_I[c] == SYM_J[c] ? 1.0 : 2.0) * ig[c] * dA[c][r];
-    out[(1 + r) * n + i] = first - 0.5 * second - 3.0 * Bw / Wf - 2.0 / 3.0 * dK[r] -
+    out[(1 + r) * n + i] = first[r] - 0.5 * second - 3.0 * Bw / Wf - 2.0 / 3.0 * dK[r] -
                            (matter ? 8.0 * M_PI * matter[(1 + r) * n + i] : 0.0);
   }
   const double det = g[0] * (g[3] * g[5] - g[4] * g[4]) - g[1] * (g[1] * g[5] - g[4] * g[2]) +
@@ -222,6 +226,18 @@ __global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__rest
   out[4 * n + i] = det - 1.0;
   out[5 * n + i] = contract(ig, A);
   for (int r = 0; r < 3; ++r) out[(6 + r) * n + i] = G[r] - contract(ig, C2[r]);
+}
+
+__global__ void __launch_bounds__(256) k_constraints(Ctx C, const double *__restrict__ us,
+                                                     const double *__restrict__ matter, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.ncell) return;
+  const int z = (int)(i % C.ax[2].n);
+  const int y = (int)((i / C.ax[2].n) % C.ax[1].n);
+  const int x = (int)(i / ((int64_t)C.ax[2].n * C.ax[1].n));
+  PointProvider<double> pv;
+  pv.setup(C, us, x, y, z);
+  constraints_eval(C, pv, us, matter, x, y, z, pv.idx, out, true);
 }
 
 // ------------------------------------------------- constraint damping
@@ -282,6 +298,31 @@ __global__ void __launch_bounds__(TX *TY *TZ) k_kappa_tile(Ctx C, const double *
   tp.setup(C, us, box, box, bx * TX, by * TY, bz * TZ, tx, ty, tz, false);  // no stash, no prefetch
   KappaTaps<P> ps{&tp, us, C.ncell}, pd{&tp, D, C.ncell}, pm{&tp, matter, C.ncell};
   kappa_eval(C, ps, pd, matter ? &pm : (KappaTaps<P> *)nullptr, us, D, matter, idx, kap, inside);
+}
+
+// The constraints on tiles: the components staged one at a time in one LDS
+// box (closure rows on face tiles), as the kappa term; the momentum
+// constraint's d_j B_r^j reads its taps from global memory.
+template <bool CL>
+__global__ void __launch_bounds__(TX *TY *TZ) k_constraints_tile(Ctx C, const double *__restrict__ us,
+                                                                const double *__restrict__ matter, double *out,
+                                                                TileSet S) {
+  using P = TileProvider<double, TX, TY, TZ, false, CL>;
+  __shared__ double box[P::NBOXP];
+  int bx, by, bz, slab;
+  tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz, &slab);
+  const int tid = threadIdx.x;
+  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
+  const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
+  const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;
+  const int yc = y < C.ax[1].n ? y : C.ax[1].n - 1;
+  const int zc = z < C.ax[2].n ? z : C.ax[2].n - 1;
+  const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
+  P tp;
+  tp.setup(C, us, box, box, bx * TX, by * TY, bz * TZ, tx, ty, tz, false);  // no stash, no prefetch
+  KappaTaps<P> ps{&tp, us, C.ncell};
+  constraints_eval(C, ps, us, matter, xc, yc, zc, idx, out, inside);
 }
 
 // The kappa term on a box one plane thick along y (the Cartoon centre
@@ -692,8 +733,32 @@ int bssn_constraints_with_matter(const bssn_params *p, const bssn_grid *grid, co
   int rc = resolve(&q, grid, &C);
   if (rc) return rc;
   if (!state || !out) return fail(BSSN_ERR_INVALID, "null buffer");
+  const hipStream_t st = (hipStream_t)stream;
+  // tiles where the stage kernels tile the grid: whole grid, no MAD blend,
+  // no reduced axis, closures inside the tiles
+  bool tiles = !C.any_shell || tiles_aligned(C);
+  for (int a = 0; a < 3; ++a)
+    tiles = tiles && C.ax[a].q == 1.0 && !C.ax[a].reduced && C.ax[a].start == 0 && C.ax[a].end == C.ax[a].n;
+  if (tiles) {
+    const dim3 block(TX * TY * TZ);
+    if (!C.any_shell) {
+      const TileSet S = tile_set(C, 0, TX, TY, TZ);
+      hipLaunchKernelGGL(k_constraints_tile<false>, dim3(S.count), block, 0, st, C, state, matter, out, S);
+      return check_hip(hipGetLastError(), "constraint tile kernel launch");
+    }
+    const TileSet lean = tile_set(C, 1, TX, TY, TZ), face = tile_set(C, 2, TX, TY, TZ);
+    if (lean.count > 0) {
+      hipLaunchKernelGGL(k_constraints_tile<false>, dim3(lean.count), block, 0, st, C, state, matter, out, lean);
+      if ((rc = check_hip(hipGetLastError(), "constraint tile kernel launch"))) return rc;
+    }
+    if (face.count > 0) {
+      hipLaunchKernelGGL(k_constraints_tile<true>, dim3(face.count), block, 0, st, C, state, matter, out, face);
+      rc = check_hip(hipGetLastError(), "constraint tile kernel launch");
+    }
+    return rc;
+  }
   const int64_t blocks = (C.ncell + 255) / 256;
-  hipLaunchKernelGGL(k_constraints, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, C, state, matter, out);
+  hipLaunchKernelGGL(k_constraints, dim3((unsigned)blocks), dim3(256), 0, st, C, state, matter, out);
   return check_hip(hipGetLastError(), "constraint kernel launch");
 }
 

@@ -317,6 +317,37 @@ def case_poisoned_guards(L):
         print(f"poisoned guards (precision {precision}): owned RHS unchanged and finite", flush=True)
 
 
+def case_constraints(L):
+    """bssn_constraints against the oracle's compute_all_constraints on the
+    tile kernels (periodic: every tile; aligned Sommerfeld: lean and face
+    tiles) and the point kernel (unaligned faces, MAD)."""
+    configs = {
+        "periodic (tiles)": (smooth_state((12, 14, 16), seed=81),
+                             OB.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=1.0 / 12, dt=0.02)),
+        "Sommerfeld aligned (lean and face tiles)": (smooth_state((24, 24, 24), seed=82), sommerfeld_params()),
+        "Sommerfeld unaligned (points)": (smooth_state((14, 16, 18), seed=83), sommerfeld_params()),
+        "MAD (points)": (smooth_state((12, 12, 12), seed=84),
+                         OB.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=1.0 / 12, dt=0.02, mad_q=0.25)),
+    }
+    for name, (v, p) in configs.items():
+        s = pack(v)
+        shape = s.shape[1:]
+        out = np.zeros((9,) + shape)
+        check(L, L.bssn_constraints(ctypes.byref(_lib.c_params(p)), ctypes.byref(_lib.c_grid(shape)), ptr(s),
+                                    ptr(out), None))
+        ref = OB.compute_all_constraints(v, p)
+        fields = [ref.hamiltonian] + list(ref.momentum) + [ref.det_gamma, ref.trace_A] + list(ref.gamma_condition)
+        worst = 0.0
+        for k, b in enumerate(fields):
+            b = np.asarray(b)
+            err = np.abs(out[k] - b).max()
+            scale = np.abs(b).max()
+            if not (err <= 1e-10 * scale or err < 1e-13):
+                raise AssertionError(f"constraints, {name}: field {k} error {err:.2e} (scale {scale:.2e})")
+            worst = max(worst, err / max(scale, 1e-300) if err >= 1e-13 else 0.0)
+        print(f"constraints, {name}: worst relative error {worst:.1e} (bar 1e-10)", flush=True)
+
+
 def _vjp(L, s, p, rbar=None, hbar=None):
     shape = s.shape[1:]
     cp, cg = _lib.c_params(p), _lib.c_grid(shape)

@@ -144,7 +144,10 @@ def test_rk4_step_api_equals_evolver_steps(bssn):
 
 
 def test_constraints_match_oracle(bssn):
-    for case in ("shift_1plog", "sommerfeld_all"):
+    # periodic and aligned Sommerfeld grids take the tile kernels (lean and
+    # face tiles), the others the point kernel
+    for case in ("shift_1plog", "sommerfeld_all", "sommerfeld_aligned", "sommerfeld_aligned_all_slabs",
+                 "sommerfeld_mixed", "mad_periodic"):
         v, p = CASES[case]()
         ref = OB.compute_all_constraints(v, p)
         got = bssn.compute_all_constraints(v, _params(bssn, p))
