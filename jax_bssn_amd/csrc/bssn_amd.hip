@@ -282,7 +282,9 @@ __global__ void __launch_bounds__(TX *TY *TZ) k_kappa_tile(Ctx C, const double *
                                                           const double *__restrict__ D,
                                                           const double *__restrict__ matter, double *kap,
                                                           TileSet S) {
-  using P = TileProvider<double, TX, TY, TZ, false, CL>;
+  // lean tiles: the closure-free provider with the padded LDS rows of the
+  // interior stage kernel (fewer bank conflicts on the mixed derivatives)
+  using P = TileProvider<double, TX, TY, TZ, false, CL, 7, !CL>;
   __shared__ double box[P::NBOXP];
   int bx, by, bz, slab;
   tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz, &slab);

@@ -313,67 +313,63 @@ template <class P, class PM>
 BSSN_HD __forceinline__ void kappa_eval(const Ctx &C, P &ps, P &pd, PM *pm, const double *__restrict__ us,
                                         const double *__restrict__ D, const double *__restrict__ matter, int64_t i,
                                         double *kap, bool write) {
+  // The fields are staged one at a time and every term is added as soon as
+  // its factors are known (A_jk paired with the derivative of gamma^jk that
+  // multiplies it, the metric last for the Christoffel term), so that few
+  // values stay live across the stagings.
   const int64_t n = C.ncell;
-  double ig[6], B[3][3], BW[3][3];
+  double ig[6];
   for (int c = 0; c < 6; ++c) ig[c] = D[c * n + i];
-  for (int r = 0; r < 3; ++r)
-    for (int j = 0; j < 3; ++j) {
-      B[r][j] = D[(6 + 3 * r + j) * n + i];
-      BW[r][j] = D[(15 + 3 * r + j) * n + i];
-    }
   const double W = us[CW * n + i], al = us[ALPHA * n + i];
   const double Wf = W > 1.0e-12 ? W : 1.0e-12;
   auto second = [&](P &pv, int inner, int outer) -> double {
     return inner == outer ? pv.d2(outer) : pv.dmix(inner, outer);
   };
-  double dW[3], dK[3], dA[6][3], dg[6][3];
+  double dW[3];
   double M[3] = {0, 0, 0}, dM[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   ps.stage_comp(CW);
   for (int j = 0; j < 3; ++j) dW[j] = ps.d1(j);
-  // -3 (B/W)_ij d_l d_j W
+  // -3 (B/W)_ij d_l d_j W; -3 B_r^j d_j W / W
   for (int j = 0; j < 3; ++j)
     for (int l = 0; l < 3; ++l) {
       const double s2 = second(ps, j, l);
-      for (int r = 0; r < 3; ++r) dM[r][l] -= 3.0 * BW[r][j] * s2;
+      for (int r = 0; r < 3; ++r) dM[r][l] -= 3.0 * D[(15 + 3 * r + j) * n + i] * s2;
     }
+  for (int r = 0; r < 3; ++r) {
+    double t3 = 0.0;
+    for (int j = 0; j < 3; ++j) t3 += D[(6 + 3 * r + j) * n + i] * dW[j];
+    M[r] -= 3.0 * t3 / Wf;
+  }
+  // -2/3 d_r K, -2/3 d_l d_r K
   ps.stage_comp(CK);
-  for (int j = 0; j < 3; ++j) dK[j] = ps.d1(j);
+  for (int r = 0; r < 3; ++r) M[r] -= 2.0 / 3.0 * ps.d1(r);
   for (int r = 0; r < 3; ++r)
     for (int l = 0; l < 3; ++l) dM[r][l] -= 2.0 / 3.0 * second(ps, r, l);
+  // -1/2 gamma^jk d_r A_jk, -1/2 gamma^jk d_l d_r A_jk - 1/2 d_l gamma^jk d_r A_jk
   for (int c = 0; c < 6; ++c) {
-    ps.stage_comp(GXX + c);
-    for (int d = 0; d < 3; ++d) dg[c][d] = ps.d1(d);
-  }
-  for (int c = 0; c < 6; ++c) {
+    const double mult = SYM_I[c] == SYM_J[c] ? 1.0 : 2.0, w = mult * ig[c];
     ps.stage_comp(AXX + c);
-    for (int d = 0; d < 3; ++d) dA[c][d] = ps.d1(d);
-    const double w = (SYM_I[c] == SYM_J[c] ? 1.0 : 2.0) * ig[c];
-    // -1/2 gamma^jk d_l d_i A_jk
-    for (int r = 0; r < 3; ++r)
+    double dA[3];
+    for (int d = 0; d < 3; ++d) dA[d] = ps.d1(d);
+    for (int r = 0; r < 3; ++r) {
+      M[r] -= 0.5 * w * dA[r];
       for (int l = 0; l < 3; ++l) dM[r][l] -= 0.5 * w * second(ps, r, l);
+    }
+    pd.stage_comp(c);
+    for (int l = 0; l < 3; ++l) {
+      const double dinv = pd.d1(l);
+      for (int r = 0; r < 3; ++r) dM[r][l] -= 0.5 * mult * dinv * dA[r];
+    }
   }
+  // d_j B_r^j, d_l d_j B_r^j - 3 d_l (B/W)_r^j d_j W
   for (int r = 0; r < 3; ++r) {
     for (int j = 0; j < 3; ++j) {
       pd.stage_comp(6 + 3 * r + j);
-      M[r] += pd.d1(j);                                // d_j B_r^j
+      M[r] += pd.d1(j);
       for (int l = 0; l < 3; ++l) dM[r][l] += second(pd, j, l);
       pd.stage_comp(15 + 3 * r + j);
       for (int l = 0; l < 3; ++l) dM[r][l] -= 3.0 * pd.d1(l) * dW[j];
     }
-  }
-  for (int c = 0; c < 6; ++c) {
-    pd.stage_comp(c);
-    const double mult = SYM_I[c] == SYM_J[c] ? 1.0 : 2.0;
-    for (int l = 0; l < 3; ++l) {
-      const double dinv = pd.d1(l);
-      for (int r = 0; r < 3; ++r) dM[r][l] -= 0.5 * mult * dinv * dA[c][r];
-    }
-  }
-  for (int r = 0; r < 3; ++r) {
-    double t2 = 0.0, t3 = 0.0;
-    for (int c = 0; c < 6; ++c) t2 += (SYM_I[c] == SYM_J[c] ? 1.0 : 2.0) * ig[c] * dA[c][r];
-    for (int j = 0; j < 3; ++j) t3 += B[r][j] * dW[j];
-    M[r] += -0.5 * t2 - 3.0 * t3 / Wf - 2.0 / 3.0 * dK[r];
   }
   if (pm) {
     // M_i - 8 pi S_i and d_l M_i - 8 pi d_l S_i (bssn/constraints.py:216-259)
@@ -384,6 +380,11 @@ BSSN_HD __forceinline__ void kappa_eval(const Ctx &C, P &ps, P &pd, PM *pm, cons
     }
   }
   // D_j M_i = d_j M_i - Gamma^k_ij M_k
+  double dg[6][3];
+  for (int c = 0; c < 6; ++c) {
+    ps.stage_comp(GXX + c);
+    for (int d = 0; d < 3; ++d) dg[c][d] = ps.d1(d);
+  }
   double C1[3][6];
   for (int a = 0; a < 3; ++a)
     for (int c = 0; c < 6; ++c) C1[a][c] = 0.5 * (dg[sym(a, SYM_I[c])][SYM_J[c]] + dg[sym(a, SYM_J[c])][SYM_I[c]] - dg[c][a]);

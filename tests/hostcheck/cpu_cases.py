@@ -154,6 +154,24 @@ def case_kappa_tiles(L):
     compare("step", evolve(L, v, p, 1), OB.rk4_step(v, p), 1e-12)
 
 
+def case_kappa_tiles_periodic(L):
+    """The kappa term on the tiles of a periodic grid (every tile lean, the
+    ordered prefetching staging), without and with matter sources."""
+    from tests.test_host_algebra import matter_sources, pack_matter
+    v = smooth_state((16, 16, 16), seed=22)
+    p = OB.BSSNParameters(eta=0.3, g=0.75, nu=0.05, dx=1.0 / 16, dt=0.02, kappa=0.3)
+    compare("rhs", rhs(L, v, p), OB.compute_bssn_rhs(v, p), 1e-10)
+    rho, S, Sij = matter_sources(v.conformal_factor.shape)
+    s = pack(v)
+    cp, cg = _lib.c_params(p), _lib.c_grid(s.shape[1:])
+    m = np.ascontiguousarray(pack_matter(rho, S, Sij))
+    out = np.zeros_like(s)
+    work = np.zeros(max(1, L.bssn_work_size(ctypes.byref(cp), ctypes.byref(cg), 0)))
+    check(L, L.bssn_rhs_with_matter(ctypes.byref(cp), ctypes.byref(cg), 0, ptr(s), ptr(m), ptr(out), ptr(work),
+                                    None))
+    compare("rhs with matter", out, OB.compute_bssn_rhs_with_matter(v, p, rho, S, Sij), 1e-10)
+
+
 def case_kappa_points(L):
     v = smooth_state((14, 15, 13), seed=21)
     p = sommerfeld_params(kappa=0.02, xl_bc=0, xr_bc=0)
