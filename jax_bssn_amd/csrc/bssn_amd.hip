@@ -277,14 +277,14 @@ struct KappaTaps {
   __device__ __forceinline__ double dmix(int a, int b) const { return tp->dmix(a, b); }
 };
 
-template <bool CL>
+template <bool CL, int CLA = 7>
 __global__ void __launch_bounds__(TX *TY *TZ) k_kappa_tile(Ctx C, const double *__restrict__ us,
                                                           const double *__restrict__ D,
                                                           const double *__restrict__ matter, double *kap,
                                                           TileSet S) {
   // lean tiles: the closure-free provider with the padded LDS rows of the
   // interior stage kernel (fewer bank conflicts on the mixed derivatives)
-  using P = TileProvider<double, TX, TY, TZ, false, CL, 7, !CL>;
+  using P = TileProvider<double, TX, TY, TZ, false, CL, CLA, !CL>;
   __shared__ double box[P::NBOXP];
   int bx, by, bz, slab;
   tile_of(S, xcd_remap((int)blockIdx.x, S.count), bx, by, bz, &slab);
@@ -483,8 +483,15 @@ int launch_kappa(const Ctx &C, const double *us, const double *matter, double *w
       hipLaunchKernelGGL(k_kappa_tile<false>, dim3(lean.count), block, 0, st, C, us, D, matter, kap, lean);
       if ((rc = check_hip(hipGetLastError(), "kappa tile kernel launch"))) return rc;
     }
+    // face tiles: one launch per slab with closures along its axis only,
+    // one for edges and corners (as the stage kernels)
     if (face.count > 0) {
-      hipLaunchKernelGGL(k_kappa_tile<true>, dim3(face.count), block, 0, st, C, us, D, matter, kap, face);
+      const TileSet f3 = tile_set(C, 3, TX, TY, TZ), f4 = tile_set(C, 4, TX, TY, TZ),
+                    f5 = tile_set(C, 5, TX, TY, TZ), f6 = tile_set(C, 6, TX, TY, TZ);
+      if (f3.count > 0) hipLaunchKernelGGL((k_kappa_tile<true, 1>), dim3(f3.count), block, 0, st, C, us, D, matter, kap, f3);
+      if (f4.count > 0) hipLaunchKernelGGL((k_kappa_tile<true, 2>), dim3(f4.count), block, 0, st, C, us, D, matter, kap, f4);
+      if (f5.count > 0) hipLaunchKernelGGL((k_kappa_tile<true, 4>), dim3(f5.count), block, 0, st, C, us, D, matter, kap, f5);
+      if (f6.count > 0) hipLaunchKernelGGL((k_kappa_tile<true, 7>), dim3(f6.count), block, 0, st, C, us, D, matter, kap, f6);
       rc = check_hip(hipGetLastError(), "kappa tile kernel launch");
     }
     return rc;
