@@ -186,12 +186,14 @@ def case_mad(L):
     compare("step", evolve(L, v, p, 1), OB.rk4_step(v, p), 1e-12)
 
 
-def _cartoon_case(full_nx, kappa):
+def _cartoon_case(full_nx, kappa, z_somm=True):
     """The smooth axisymmetric state of the GPU Cartoon tests with z
-    Sommerfeld faces and an O(1e-2) trace-free extrinsic curvature."""
+    Sommerfeld (or periodic) faces and an O(1e-2) trace-free extrinsic
+    curvature."""
     from tests.test_oracle_cartoon import _set_positive, full_and_compact
     x, y, z, dx, compact, cp = full_and_compact(full_nx)
-    cp = cp._replace(nu=0.05, eta=1.0, g=0.75, kappa=kappa, dt=0.2 * dx, zl_bc=1, zr_bc=1)
+    zb = 1 if z_somm else 0
+    cp = cp._replace(nu=0.05, eta=1.0, g=0.75, kappa=kappa, dt=0.2 * dx, zl_bc=zb, zr_bc=zb)
     nr = compact.lapse.shape[0] - 4
     rho = (np.arange(nr) + 0.5) * dx
     R, Z = np.meshgrid(rho, z, indexing="ij")
@@ -227,6 +229,10 @@ def case_cartoon_region(L):
     # full_nx = 32: 16 radial points and 32 z points on 16 x 1 x 16 region tiles
     v, p = _cartoon_case(32, 0.02)
     compare("compact rhs (region tiles)", compact_rhs(L, pack(v), p), OC.compute_axisymmetric_rhs(v, p), 1e-10)
+    # periodic z: the region tiles' LDS halos wrap
+    v, p = _cartoon_case(32, 0.02, z_somm=False)
+    compare("compact rhs (region tiles, periodic z)", compact_rhs(L, pack(v), p), OC.compute_axisymmetric_rhs(v, p),
+            1e-10)
 
 
 def case_cartoon_points(L):
