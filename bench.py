@@ -76,6 +76,47 @@ def time_steps(J, state, params, steps, warmup, precision):
     return e0.elapsed_time(e1) / 1e3, ev
 
 
+def kappa_step(J, state, params, steps, warmup, ms_kappa0, kappa=0.002):
+    """The same puncture step with constraint damping kappa = 0.002 (the
+    momentum-constraint term of the A RHS): ms per step and the ratio to the
+    kappa = 0 step."""
+    import torch
+    t, ev = time_steps(J, state.clone(), params._replace(kappa=kappa), steps, warmup, "f64")
+    finite = bool(torch.isfinite(ev.state).all())
+    del ev
+    ms = 1e3 * t / steps
+    n3 = state[0].numel()
+    return {"kappa": kappa, "ms_per_step": ms, "value": n3 * steps / t, "unit": "cell-updates/s",
+            "ratio_vs_kappa0": ms / ms_kappa0, "finite": finite}
+
+
+def constraints_timing(J, state, params, reps=5):
+    """bssn_constraints (H, M_i, det - 1, tr A, Gamma constraint) on the
+    workload state: device ms per call and the L2 Hamiltonian constraint."""
+    import ctypes
+
+    import torch
+    from jax_bssn_amd import _lib
+    from jax_bssn_amd.time_evolve import _ptr, stream_handle
+    n3 = tuple(state.shape[1:])
+    out = torch.empty((9,) + n3, dtype=torch.float64, device=state.device)
+    lib = _lib.load()
+    cp, cg = _lib.c_params(params), _lib.c_grid(n3)
+
+    def call():
+        _lib.check(lib.bssn_constraints(ctypes.byref(cp), ctypes.byref(cg), _ptr(state), _ptr(out), stream_handle()))
+    call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    return {"ms_per_call": e0.elapsed_time(e1) / reps, "calls": reps,
+            "hamiltonian_l2": float(torch.sqrt(torch.mean(out[0] * out[0])))}
+
+
 # the interior tile kernel, and the face-tile kernels (one per slab with
 # closures along its axis, CLA = 1, 2, 4, and the edges and corners, CLA = 7)
 KERNEL_NAMES = {("f64", 4): "k_stage_tile<double, 4, 8, 8, false, false, 7>",
@@ -484,6 +525,8 @@ def main():
     ap.add_argument("--no-cartoon", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--no-fmr", action="store_true")
+    ap.add_argument("--no-kappa", action="store_true",
+                    help="skip the kappa = 0.002 step and the constraint-kernel legs")
     ap.add_argument("--accuracy-steps", type=int, default=50,
                     help="steps of the FP64 / FP32-RHS comparison run (0: none)")
     ap.add_argument("--adjoint-size", type=int, default=0,
@@ -526,6 +569,9 @@ def main():
             e32 = J.Evolver(state.clone(), params, precision="f32rhs").step(args.accuracy_steps)
             extra["f32rhs"]["long_run"] = f32_accuracy(e64.state, e32.state, params, args.accuracy_steps)
             del e64, e32
+    if not args.no_kappa:
+        extra["kappa_step"] = kappa_step(J, state, params, args.steps, args.warmup, 1e3 * t64 / args.steps)
+        extra["constraints"] = constraints_timing(J, final64, params)
     del final64
     torch.cuda.synchronize()
     line = {
