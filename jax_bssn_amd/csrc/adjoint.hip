@@ -7,10 +7,9 @@
 // bssn_rhs_vjp forms state_bar = J_F(state)^T rhs_bar + J_H(state)^T h_bar
 // in three steps:
 //  * the stage kernels in recording mode (record_f64.hip): the features of
-//    every point (values, first and second derivatives, upwind derivatives
-//    and Kreiss-Oliger sums of every component, with the reference's
-//    stencil rows: closures, MAD, upwind side by the sign of the shift at
-//    the point) as NFEAT grid functions;
+//    every point (values, first, second and upwind derivatives of every
+//    component, with the reference's stencil rows: closures, MAD, upwind
+//    side by the sign of the shift at the point) as NFEAT grid functions;
 //  * k_vjp_point: the adjoints of those features (adjoint_point.h);
 //  * k_vjp_transpose: at every point q, each component's adjoint is the
 //    sum, over every point p whose stencils read q, of the row coefficient

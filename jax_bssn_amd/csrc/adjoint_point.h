@@ -35,7 +35,8 @@ constexpr int F_VAL = 0;   // 21 point values
 constexpr int F_D1 = 21;   // d_a u_c, 21 x 3 (of A only on Sommerfeld face points)
 constexpr int F_D2 = 84;   // 11 x 6: d_a d_b of shift (3), metric (6), lapse, W; slot sym(a, b)
 constexpr int F_UP = 150;  // 21 x 3 upwind derivatives
-constexpr int F_KO = 213;  // 21 Kreiss-Oliger sums (with the nu / 64 dx^-1 factor)
+constexpr int F_KO = 213;  // 21 Kreiss-Oliger sums (with the nu / 64 dx^-1 factor; not recorded by the
+                           // stage kernels, whose adjoint buffer holds the effective rbar here)
 constexpr int NFEAT = 234;
 
 // slot of component c among the 11 with second derivatives, -1 otherwise

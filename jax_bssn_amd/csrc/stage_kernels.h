@@ -53,12 +53,14 @@ __device__ __forceinline__ void load_point(const Ctx &C, const double *__restric
 
 // ------------------------------------------------------ feature recording
 
-// A stencil provider that also stores every stencil value the per-point
-// RHS asks of it -- the features of adjoint_point.h: first, second, upwind
-// derivatives and Kreiss-Oliger sums of the visited component, and on a
-// Sommerfeld face point the first derivatives of the face RHS -- into
-// feat[slot * n + idx].  Run through the stage kernels with an emitter that
-// keeps nothing, the compiler drops the algebra after the stencils.
+// A stencil provider that also stores the stencil values the per-point
+// RHS asks of it -- the features of adjoint_point.h: first, second and
+// upwind derivatives of the visited component, and on a Sommerfeld face
+// point the first derivatives of the face RHS -- into feat[slot * n + idx].
+// The Kreiss-Oliger sums enter the RHS linearly and their adjoint needs no
+// feature, so they are not stored.  Run through the stage kernels with an
+// emitter that keeps nothing, the compiler drops the algebra after the
+// stencils (and the dissipation stencils altogether).
 template <class P>
 struct Recorder : P {
   // (names of their own: the providers have members such as idx)
@@ -98,11 +100,6 @@ struct Recorder : P {
   __device__ __forceinline__ auto upw(int a, C coeff) const {
     const auto v = P::upw(a, coeff);
     keep(F_UP + 3 * vc + a, (double)v);
-    return v;
-  }
-  __device__ __forceinline__ auto ko() const {
-    const auto v = P::ko();
-    keep(F_KO + vc, (double)v);
     return v;
   }
   __device__ __forceinline__ auto somm(int comp, double uc) const {
