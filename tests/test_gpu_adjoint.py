@@ -171,3 +171,50 @@ def test_reverse_gradient_matches_forward_mode_on_a_puncture(J):
     assert abs(float(rev) - float(fwd)) <= 1e-12 * abs(float(fwd))
     rel = (g_rev - g_fwd).abs() / g_fwd.abs()
     assert float(rel.max()) <= 1e-8, rel
+
+
+def test_reverse_gradient_matches_the_oracle_complex_step(J):
+    """The same objective and gradient as above, against the oracle itself:
+    the puncture initial data, 8 RK4 steps, H and its L2 norm evaluated by
+    the FP64 oracle on complex parameters (M + i h, eps + i h), whose
+    imaginary parts are the exact derivatives -- the stand-in for jax.grad
+    of the reference.  Value to 1e-10, gradient to 1e-8 (the task's bar)."""
+    from jax_bssn_amd.adjoint import HamiltonianL2Reverse
+    from jax_bssn_amd.tangent import puncture_initial
+    import bench
+    n, L, steps = 24, 8.0, 8
+    dx = 2 * L / n
+    x = -L + (np.arange(n) + 0.5) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    centre, width = (0.5, -0.3, 0.2), 2.0
+    p = bench.puncture_params(J.BSSNParameters, dx, x[0])
+    theta = torch.tensor([1.0, 1e-3], dtype=torch.float64, requires_grad=True)
+    rev = HamiltonianL2Reverse.apply(theta, puncture_initial(X, Y, Z, width=width, lapse_centre=centre), p, steps, 4)
+    rev.backward()
+    g_rev = theta.grad.clone().numpy()
+    op = OB.BSSNParameters(**p._asdict())
+
+    def l2(M, eps):
+        r = np.sqrt(X * X + Y * Y + Z * Z)
+        W = (1.0 + M / (2.0 * r)) ** -2
+        gauss = np.exp(-((X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2) / width ** 2)
+        zero = np.zeros((n, n, n), dtype=complex)
+        g = np.zeros((3, 3, n, n, n), dtype=complex)
+        g[0, 0] = g[1, 1] = g[2, 2] = 1.0
+        v = OB.BSSNVariables(g, W + zero, np.zeros((3, 3, n, n, n), dtype=complex), zero.copy(),
+                             np.zeros((3, n, n, n), dtype=complex), W + eps * gauss + zero,
+                             np.zeros((3, n, n, n), dtype=complex))
+        for _ in range(steps):
+            v = OB.rk4_step(v, op)
+        H = OB.compute_hamiltonian_constraint(v, op)
+        return np.sqrt(np.mean(H * H))
+
+    h = 1e-30
+    print("[adjoint vs oracle complex step] puncture 24^3, 8 steps", flush=True)
+    lM = l2(1.0 + 1j * h, 1e-3)
+    le = l2(1.0, 1e-3 + 1j * h)
+    g_ora = np.array([np.imag(lM) / h, np.imag(le) / h])
+    print("value", float(rev), float(np.real(lM)), "gradient", g_rev.tolist(), g_ora.tolist(), flush=True)
+    assert abs(float(rev) - float(np.real(lM))) <= 1e-10 * abs(float(np.real(lM)))
+    rel = np.abs(g_rev - g_ora) / np.abs(g_ora)
+    assert float(rel.max()) <= 1e-8, rel
