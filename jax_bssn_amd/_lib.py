@@ -50,7 +50,7 @@ class CTileLayout(ctypes.Structure):
 class CBatchPlan(ctypes.Structure):
     """bssn_batch_plan of include/bssn_amd.h."""
     _fields_ = [("ninst", ctypes.c_int32), ("total_tiles", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("tile_variant", ctypes.c_int32), ("grid_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("tile_variant", ctypes.c_int32), ("grid_blocks", ctypes.c_int32), ("face_tiles", ctypes.c_int32),
                 ("total_cells", ctypes.c_int64), ("desc_offset", ctypes.c_int64), ("prefix_offset", ctypes.c_int64),
                 ("proj_offset", ctypes.c_int64)]
 

@@ -4,9 +4,10 @@ the tile shape and the persistent grid size.
 
 The reference evolves one grid per call (evolution/time_evolve.py:144-217);
 ``BatchEvolver.step`` applies the same ``rk4_step`` to every instance of the
-batch.  Instances must have periodic faces, mad_q = 1 and kappa = 0 (the
-batched gauge-wave configuration); other configurations run through
-``Evolver`` one grid at a time.
+batch.  Instances may have periodic or Sommerfeld faces (Sommerfeld faces on
+the tile grid of the tile shape used: extents that are multiples of 4 or 8
+along those axes), mad_q = 1 and kappa = 0; other configurations run
+through ``Evolver`` one grid at a time.
 """
 
 import ctypes
@@ -49,7 +50,8 @@ class BatchEvolver:
         self.work = torch.empty(size, dtype=torch.float64, device=dev)
         self.plan = _lib.CBatchPlan()
         self.graph = None
-        key = (self.shapes, precision)
+        key = (self.shapes, tuple((p.xl_bc, p.xr_bc, p.yl_bc, p.yr_bc, p.zl_bc, p.zr_bc) for p in self.params),
+               precision)
         self.tuning = {"source": "given"}
         if tile_variant is None or grid_blocks is None:
             if autotune and key in _TUNED:
@@ -108,13 +110,18 @@ class BatchEvolver:
 
     def autotune(self, reps=2):
         """Time every tile shape with a few persistent grid sizes on scratch
-        copies of the states; return the fastest (tile_variant, grid_blocks)."""
+        copies of the states; return the fastest (tile_variant, grid_blocks).
+        Tile shapes whose grid some instance's Sommerfeld faces are not on
+        are skipped."""
         saved = [s.clone() for s in self.states]
         cus = _cu_count()
-        best, best_t = (0, 2 * cus), float("inf")
+        best, best_t = None, float("inf")
         for v in TILE_VARIANTS:
             for blocks in (cus, 2 * cus, 4 * cus, 1 << 20):
-                self.prepare(v, blocks)
+                try:
+                    self.prepare(v, blocks)
+                except NotImplementedError:
+                    break
                 self._step_once()
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
@@ -126,6 +133,8 @@ class BatchEvolver:
                     best, best_t = (v, blocks), t
         for s, v in zip(self.states, saved):
             s.copy_(v)
+        if best is None:
+            raise NotImplementedError("no tile shape of the batched launch fits the Sommerfeld faces of this batch")
         self.tuning = {"source": "autotuned", "step_seconds": best_t}
         return best
 

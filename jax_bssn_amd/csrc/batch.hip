@@ -1,18 +1,21 @@
-// Batched evolution: many independent periodic evolutions of uneven grid
-// sizes advanced by one persistent launch per RK4 stage.
+// Batched evolution: many independent evolutions of uneven grid sizes
+// advanced by persistent launches per RK4 stage.
 //
 // A prepared batch lives in a caller-owned work buffer: per instance its RK4
 // buffers and four stage descriptors (resolved parameters, stage buffers and
 // weights, tile counts, first tile), plus prefix sums of tiles and points.
-// Each stage is one launch of a fixed grid of workgroups that walk the
-// global tile list (tile t of instance i at prefix[i] + local index) and run
-// the same per-point RHS and in-register stage update as the single-grid
-// tile kernel.  A step is a batched projection plus the four stage launches,
-// with no host work in between, so it can be captured in a HIP graph.
+// Each stage is a launch of a fixed grid of workgroups that walk the global
+// list of lean tiles (tile t of instance i at prefix[i] + local index) and
+// run the same per-point RHS and in-register stage update as the
+// single-grid tile kernel, and, when some instance has Sommerfeld faces, a
+// second such launch over the global list of face tiles with the
+// closure-capable provider (one-sided rows and the radial face RHS).  A
+// step is a batched projection plus the stage launches, with no host work
+// in between, so it can be captured in a HIP graph.
 //
-// Restricted to what the batched configuration needs: periodic faces,
-// mad_q = 1 and kappa = 0 (every other configuration runs through the
-// single-grid entry points).
+// Restricted to periodic or Sommerfeld faces on the tile grid of the
+// chosen tile shape, mad_q = 1 and kappa = 0 (every other configuration
+// runs through the single-grid entry points).
 
 #include <hip/hip_runtime.h>
 
@@ -27,6 +30,7 @@
 #include "rhs_point.h"
 #include "stage_common.h"
 #include "stencil_providers.h"
+#include "tiles.h"
 
 namespace bssn {
 namespace {
@@ -35,8 +39,8 @@ struct BatchDesc {
   Ctx C;
   StageOut O;
   const double *us;
-  int nb[3];        // tiles per axis
-  int tile_begin;   // first global tile of the instance
+  TileSet lean, face;          // the instance's tiles without / with closure points
+  int lean_begin, face_begin;  // first global lean / face tile of the instance
 };
 
 struct ProjDesc {
@@ -51,27 +55,34 @@ __host__ __device__ constexpr int variant_tile(int v, int a) {
   return v == 0 ? (a == 0 ? 4 : 8) : (v == 1 ? (a == 2 ? 4 : 8) : (a == 1 ? 4 : 8));
 }
 
-// Compiled for the occupancy of the single-grid interior tile kernel (one
-// wave per SIMD in FP64, two in FP32).
-template <typename T, int TX, int TY, int TZ>
-__global__ void __launch_bounds__(TX *TY *TZ, (sizeof(T) == sizeof(double) ? 1 : 2))
+// last instance whose first tile (prefix) is <= t (block-uniform)
+__device__ __forceinline__ int batch_instance(const int *__restrict__ prefix, int ninst, int t) {
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The lean tiles, compiled for the occupancy of the single-grid interior
+// tile kernel (one wave per SIMD in FP64, two in FP32); the face tiles (CL)
+// with the closure-capable provider, as the single-grid face tiles.
+template <typename T, int TX, int TY, int TZ, bool CL>
+__global__ void __launch_bounds__(TX *TY *TZ, (!CL && sizeof(T) == sizeof(double) ? 1 : 2))
     k_batch_tile(const BatchDesc *__restrict__ D, const int *__restrict__ prefix, int ninst, int total) {
-  using P = TileProvider<T, TX, TY, TZ, false, false, 7, sizeof(T) == sizeof(double)>;
+  using P = TileProvider<T, TX, TY, TZ, false, CL, 7, !CL && sizeof(T) == sizeof(double)>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   const int tid = threadIdx.x;
   const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    int lo = 0, hi = ninst - 1;
-    while (lo < hi) {  // last instance whose first tile is <= t (block-uniform)
-      const int mid = (lo + hi + 1) >> 1;
-      if (prefix[mid] <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    const BatchDesc &d = D[lo];
+    const BatchDesc &d = D[batch_instance(prefix, ninst, t)];
     const Ctx &C = d.C;
-    const int k = t - d.tile_begin;
-    const int bz = k % d.nb[2], by = (k / d.nb[2]) % d.nb[1], bx = k / (d.nb[2] * d.nb[1]);
+    int bx, by, bz;
+    if (CL) tile_of(d.face, t - d.face_begin, bx, by, bz);
+    else tile_of(d.lean, t - d.lean_begin, bx, by, bz);
     const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
     const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
     const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;
@@ -86,7 +97,7 @@ __global__ void __launch_bounds__(TX *TY *TZ, (sizeof(T) == sizeof(double) ? 1 :
 #pragma unroll
       for (int c = 0; c < NCOMP; ++c) u[c] = g[c * C.ncell + idx];
     }
-    Emitter<LdsStash<TX * TY * TZ>> out(C, d.O, idx, inside, false, pv.st);
+    Emitter<LdsStash<TX * TY * TZ>> out(C, d.O, idx, inside, pv.face(), pv.st);
     bssn_rhs_point<T>(pv, C, u, out);
     __syncthreads();  // the next tile restages the LDS box
   }
@@ -128,7 +139,8 @@ BatchLayout batch_layout(const bssn_grid *grids, int n) {
   BatchLayout L;
   L.desc = 0;
   L.prefix = L.desc + dwords((int64_t)4 * n * sizeof(BatchDesc));
-  L.proj = L.prefix + dwords((int64_t)(n + 1) * sizeof(int));
+  // lean-tile prefix sums (n + 1), then face-tile prefix sums (n + 1)
+  L.proj = L.prefix + dwords((int64_t)2 * (n + 1) * sizeof(int));
   L.bufs = L.proj + dwords((int64_t)n * sizeof(ProjDesc));
   int64_t cells = 0;
   for (int i = 0; i < n; ++i) cells += (int64_t)grids[i].nx * grids[i].ny * grids[i].nz;
@@ -136,18 +148,18 @@ BatchLayout batch_layout(const bssn_grid *grids, int n) {
   return L;
 }
 
-template <typename T>
+template <typename T, bool CL>
 int launch_batch_t(int variant, const BatchDesc *D, const int *prefix, int ninst, int total, int blocks,
                    hipStream_t st) {
   switch (variant) {
     case 0:
-      hipLaunchKernelGGL((k_batch_tile<T, 4, 8, 8>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      hipLaunchKernelGGL((k_batch_tile<T, 4, 8, 8, CL>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
       break;
     case 1:
-      hipLaunchKernelGGL((k_batch_tile<T, 8, 8, 4>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 8, 4, CL>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
       break;
     default:
-      hipLaunchKernelGGL((k_batch_tile<T, 8, 4, 8>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 4, 8, CL>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
       break;
   }
   return check_hip(hipGetLastError(), "batched stage kernel launch");
@@ -176,23 +188,35 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
   if (grid_blocks < 1) return fail(BSSN_ERR_INVALID, "grid_blocks must be positive");
   const BatchLayout L = batch_layout(grids, n);
   std::vector<BatchDesc> descs((size_t)4 * n);
-  std::vector<int> prefix((size_t)n + 1);
+  std::vector<int> prefix((size_t)2 * (n + 1));  // lean, then face
   std::vector<ProjDesc> proj((size_t)n);
-  int64_t tiles = 0, cells = 0, buf_off = L.bufs;
+  int64_t tiles = 0, ftiles = 0, cells = 0, buf_off = L.bufs;
+  const int t[3] = {variant_tile(tile_variant, 0), variant_tile(tile_variant, 1), variant_tile(tile_variant, 2)};
   for (int i = 0; i < n; ++i) {
     Ctx C;
     char err[512] = {0};
     int rc = resolve(&params[i], &grids[i], &C, err, sizeof err);
     if (rc) return fail(rc, "instance %d: %s", i, err);
     if (!states[i]) return fail(BSSN_ERR_INVALID, "instance %d: null state", i);
-    if (C.any_shell || C.kappa != 0.0 || C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0)
+    if (C.kappa != 0.0 || C.ax[0].q != 1.0 || C.ax[1].q != 1.0 || C.ax[2].q != 1.0)
+      return fail(BSSN_ERR_UNSUPPORTED, "instance %d: the batched launch takes mad_q = 1 and kappa = 0", i);
+    bool aligned = true, whole = true;
+    for (int a = 0; a < 3; ++a) {
+      const Axis &A = C.ax[a];
+      whole = whole && A.start == 0 && A.end == A.n && !A.reduced;
+      if ((A.lbc || A.rbc) && (A.start % t[a] != 0 || A.end % t[a] != 0)) aligned = false;
+    }
+    if (C.any_shell && !(aligned && whole))
       return fail(BSSN_ERR_UNSUPPORTED,
-                  "instance %d: the batched launch takes periodic faces, mad_q = 1 and kappa = 0", i);
-    int nb[3];
-    for (int a = 0; a < 3; ++a) nb[a] = (C.ax[a].n + variant_tile(tile_variant, a) - 1) / variant_tile(tile_variant, a);
-    const int64_t nt = (int64_t)nb[0] * nb[1] * nb[2];
-    if (tiles + nt >= ((int64_t)1 << 31)) return fail(BSSN_ERR_INVALID, "batch has too many tiles");
+                  "instance %d: Sommerfeld faces in a batch must lie on the %d x %d x %d tile grid (whole grids)", i,
+                  t[0], t[1], t[2]);
+    const TileSet lean = tile_set(C, C.any_shell ? 1 : 0, t[0], t[1], t[2]);
+    TileSet face = tile_set(C, 2, t[0], t[1], t[2]);
+    if (!C.any_shell) face.count = 0;
+    if (tiles + lean.count >= ((int64_t)1 << 31) || ftiles + face.count >= ((int64_t)1 << 31))
+      return fail(BSSN_ERR_INVALID, "batch has too many tiles");
     prefix[i] = (int)tiles;
+    prefix[(size_t)n + 1 + i] = (int)ftiles;
     double *u = states[i];
     double *ustar = work + buf_off, *ua = ustar + NCOMP * C.ncell, *ub = ua + NCOMP * C.ncell;
     buf_off += 3 * NCOMP * C.ncell;
@@ -205,14 +229,18 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
       d.O = StageOut{s == 3 ? nullptr : u, ustar, outs[s], nullptr, s + 1, 0, nullptr, nullptr};
       stage_update(d.O, C.dt);
       d.us = ins[s];
-      for (int a = 0; a < 3; ++a) d.nb[a] = nb[a];
-      d.tile_begin = (int)tiles;
+      d.lean = lean;
+      d.face = face;
+      d.lean_begin = (int)tiles;
+      d.face_begin = (int)ftiles;
     }
     proj[i] = ProjDesc{u, C.ncell, cells};
-    tiles += nt;
+    tiles += lean.count;
+    ftiles += face.count;
     cells += C.ncell;
   }
   prefix[n] = (int)tiles;
+  prefix[(size_t)2 * n + 1] = (int)ftiles;
   hipStream_t st = (hipStream_t)stream;
   char *base = (char *)work;
   int rc = check_hip(hipMemcpyAsync(base + L.desc * 8, descs.data(), descs.size() * sizeof(BatchDesc),
@@ -226,6 +254,7 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
   if (rc) return rc;
   plan->ninst = n;
   plan->total_tiles = (int32_t)tiles;
+  plan->face_tiles = (int32_t)ftiles;
   plan->precision = precision;
   plan->tile_variant = tile_variant;
   plan->grid_blocks = grid_blocks;
@@ -238,7 +267,8 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
 
 int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream) {
   if (!plan || !work) return fail(BSSN_ERR_INVALID, "null argument");
-  if (plan->ninst < 1 || plan->total_tiles < 1 || plan->grid_blocks < 1 || plan->tile_variant < 0 ||
+  if (plan->ninst < 1 || plan->total_tiles < 0 || plan->face_tiles < 0 ||
+      plan->total_tiles + plan->face_tiles < 1 || plan->grid_blocks < 1 || plan->tile_variant < 0 ||
       plan->tile_variant >= BATCH_VARIANTS)
     return fail(BSSN_ERR_INVALID, "batch plan is not prepared");
   hipStream_t st = (hipStream_t)stream;
@@ -248,12 +278,20 @@ int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream) {
   const int64_t pblocks = (plan->total_cells + 255) / 256;
   hipLaunchKernelGGL(k_batch_project, dim3((unsigned)pblocks), dim3(256), 0, st, PD, plan->ninst, plan->total_cells);
   int rc = check_hip(hipGetLastError(), "batched projection launch");
-  const int blocks = plan->grid_blocks < plan->total_tiles ? plan->grid_blocks : plan->total_tiles;
+  const int *fprefix = prefix + plan->ninst + 1;
+  const int lblocks = plan->grid_blocks < plan->total_tiles ? plan->grid_blocks : plan->total_tiles;
+  const int fblocks = plan->grid_blocks < plan->face_tiles ? plan->grid_blocks : plan->face_tiles;
+  const bool f64 = plan->precision == BSSN_PREC_F64;
   for (int s = 0; s < 4 && !rc; ++s) {
+    // lean and face tiles write disjoint points of the stage's outputs
     const BatchDesc *Ds = D + (size_t)s * plan->ninst;
-    rc = plan->precision == BSSN_PREC_F64
-             ? launch_batch_t<double>(plan->tile_variant, Ds, prefix, plan->ninst, plan->total_tiles, blocks, st)
-             : launch_batch_t<float>(plan->tile_variant, Ds, prefix, plan->ninst, plan->total_tiles, blocks, st);
+    const int v = plan->tile_variant, ni = plan->ninst;
+    if (lblocks > 0)
+      rc = f64 ? launch_batch_t<double, false>(v, Ds, prefix, ni, plan->total_tiles, lblocks, st)
+               : launch_batch_t<float, false>(v, Ds, prefix, ni, plan->total_tiles, lblocks, st);
+    if (!rc && fblocks > 0)
+      rc = f64 ? launch_batch_t<double, true>(v, Ds, fprefix, ni, plan->face_tiles, fblocks, st)
+               : launch_batch_t<float, true>(v, Ds, fprefix, ni, plan->face_tiles, fblocks, st);
   }
   return rc;
 }

@@ -289,15 +289,19 @@ def case_tangent_points(L):
 
 
 def case_batch(L):
-    """Two uneven periodic gauge-wave-like instances advanced by the batched
-    persistent launch against the same steps one grid at a time."""
+    """Two uneven periodic gauge-wave-like instances and a Sommerfeld one
+    (lean and face tiles) advanced by the batched persistent launches against
+    the same steps one grid at a time."""
     from jax_bssn_amd import _lib as LB
-    shapes = ((16, 16, 16), (24, 16, 32))
+    shapes = ((16, 16, 16), (24, 16, 32), (24, 16, 32))
     states, params = [], []
     for k, sh in enumerate(shapes):
         v = smooth_state(sh, seed=40 + k, amp=0.3)
-        params.append(OB.BSSNParameters(eta=0.0, g=0.0, nu=0.1, dx=1.0 / sh[0], dt=0.25 / sh[0], zero_shift=1,
-                                        gauge=0))
+        if k < 2:
+            params.append(OB.BSSNParameters(eta=0.0, g=0.0, nu=0.1, dx=1.0 / sh[0], dt=0.25 / sh[0], zero_shift=1,
+                                            gauge=0))
+        else:
+            params.append(sommerfeld_params())
         states.append(pack(v))
     ref = [evolve(L, unpack(s), p, 2) for s, p in zip(states, params)]
     n = len(shapes)

@@ -6,6 +6,9 @@
 * all tile-shape variants give the same FP64 states;
 * a HIP-graph-captured step, replayed three times with eager work and
   synchronisations in between, equals the eager batched step every time;
+* a batch mixing periodic gauge waves and Sommerfeld instances (lean and
+  face tiles of every tile shape) equals the single-grid evolutions and the
+  oracle;
 * configurations outside the batched path raise NotImplementedError.
 """
 
@@ -101,10 +104,56 @@ def test_autotuner_picks_a_valid_variant(J):
 def test_unsupported_configurations_raise(J):
     from jax_bssn_amd.batch import BatchEvolver
     states, params, _ = gauge_batch(J, (16,), seed=1)
-    with pytest.raises(NotImplementedError, match="periodic"):
-        BatchEvolver(states, [params[0]._replace(xl_bc=1, xr_bc=1)], tile_variant=0, grid_blocks=64)
+    # 20 points along y with Sommerfeld y faces: not on the 8-point tile rows
+    states20, params20, _ = gauge_batch(J, (20,), seed=1)
+    with pytest.raises(NotImplementedError, match="tile grid"):
+        BatchEvolver(states20, [params20[0]._replace(yl_bc=1, yr_bc=1)], tile_variant=0, grid_blocks=64)
     with pytest.raises(NotImplementedError, match="kappa"):
         BatchEvolver(states, [params[0]._replace(kappa=0.1)], tile_variant=0, grid_blocks=64)
+    with pytest.raises(NotImplementedError, match="mad_q"):
+        BatchEvolver(states, [params[0]._replace(mad_q=0.5)], tile_variant=0, grid_blocks=64)
+
+
+def sommerfeld_instances(J, shapes, seed=50):
+    from jax_bssn_amd.variables import pack
+    from tests.parity_utils import smooth_state
+    states, params, ref = [], [], []
+    for k, sh in enumerate(shapes):
+        v = smooth_state(sh, seed=seed + k)
+        p = OB.BSSNParameters(eta=2.0, g=0.75, nu=0.1, dx=0.1, dt=0.025, zero_shift=0, gauge=1, xl_bc=1, xr_bc=1,
+                              yl_bc=1, yr_bc=1, zl_bc=1, zr_bc=1, x_min=-0.75, y_min=-0.8, z_min=-0.83)
+        states.append(pack(v))
+        params.append(J.BSSNParameters(**p._asdict()))
+        ref.append((v, p))
+    return states, params, ref
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_batch_with_sommerfeld_instances(J, variant):
+    """Periodic gauge waves and Sommerfeld instances (faces on every tile
+    shape's grid) in one batch: lean tiles and face tiles in their own
+    persistent launches; each instance equals its single-grid evolution
+    (1e-14) and a Sommerfeld one matches the oracle step by step (1e-12)."""
+    from jax_bssn_amd.batch import BatchEvolver
+    gs, gp, _ = gauge_batch(J, (16, 24), seed=3)
+    ss, sp, sref = sommerfeld_instances(J, ((24, 24, 24), (16, 16, 32), (32, 24, 16)))
+    states, params = gs + ss, gp + sp
+    print(f"[batch sommerfeld] variant={variant}", flush=True)
+    b = BatchEvolver([s.clone() for s in states], params, tile_variant=variant, grid_blocks=256)
+    assert b.plan.face_tiles > 0
+    singles = [J.Evolver(s.clone(), p) for s, p in zip(states, params)]
+    v0, p0 = sref[0]
+    for step in range(3):
+        b.step(1)
+        for e in singles:
+            e.step(1)
+        v0 = OB.rk4_step(v0, p0)
+        errs = rel_errors(b.variables(np.zeros(1))[2], v0)
+        assert max(errs.values()) <= 1e-12, (step, errs)
+    for i, (bs, e) in enumerate(zip(b.states, singles)):
+        d = float((bs - e.state).abs().max() / e.state.abs().max())
+        print("batch (sommerfeld) vs single", i, f"{d:.1e}", flush=True)
+        assert d <= 1e-14
 
 
 def test_f32rhs_batch_matches_single_grid_f32rhs(J):
