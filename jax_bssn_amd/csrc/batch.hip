@@ -39,8 +39,9 @@ struct BatchDesc {
   Ctx C;
   StageOut O;
   const double *us;
-  TileSet lean, face;          // the instance's tiles without / with closure points
-  int lean_begin, face_begin;  // first global lean / face tile of the instance
+  TileSet lean;     // the instance's tiles without closure points
+  TileSet face[4];  // with closures along x, y, z only, and edges and corners
+  int lean_begin, face_begin[4];  // first global tile of the instance in each list
 };
 
 struct ProjDesc {
@@ -66,23 +67,37 @@ __device__ __forceinline__ int batch_instance(const int *__restrict__ prefix, in
   return lo;
 }
 
+// The tile lists of a batch: lean tiles, then the face tiles with closures
+// along x, y, z only and the edge and corner tiles (CLA = 1, 2, 4, 7).
+__host__ __device__ constexpr int face_list(int cla) { return cla == 1 ? 0 : (cla == 2 ? 1 : (cla == 4 ? 2 : 3)); }
+
 // The lean tiles, compiled for the occupancy of the single-grid interior
 // tile kernel (one wave per SIMD in FP64, two in FP32); the face tiles (CL)
-// with the closure-capable provider, as the single-grid face tiles.
-template <typename T, int TX, int TY, int TZ, bool CL>
+// with the closure-capable provider and the wave orientation of the
+// single-grid face tiles.  Each walks its global list, whose length is the
+// last entry of its prefix sums.
+template <typename T, int TX, int TY, int TZ, bool CL, int CLA>
 __global__ void __launch_bounds__(TX *TY *TZ, (!CL && sizeof(T) == sizeof(double) ? 1 : 2))
-    k_batch_tile(const BatchDesc *__restrict__ D, const int *__restrict__ prefix, int ninst, int total) {
-  using P = TileProvider<T, TX, TY, TZ, false, CL, 7, !CL && sizeof(T) == sizeof(double)>;
+    k_batch_tile(const BatchDesc *__restrict__ D, const int *__restrict__ prefix, int ninst) {
+  using P = TileProvider<T, TX, TY, TZ, false, CL, CLA, !CL && sizeof(T) == sizeof(double)>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   const int tid = threadIdx.x;
-  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+  const int total = prefix[ninst];
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     const BatchDesc &d = D[batch_instance(prefix, ninst, t)];
     const Ctx &C = d.C;
-    int bx, by, bz;
-    if (CL) tile_of(d.face, t - d.face_begin, bx, by, bz);
+    int bx, by, bz, slab = 0;
+    if (CL) tile_of(d.face[face_list(CLA)], t - d.face_begin[face_list(CLA)], bx, by, bz, &slab);
     else tile_of(d.lean, t - d.lean_begin, bx, by, bz);
+    int tx, ty, tz;
+    if (!CL || slab == 0) {
+      tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TY * TZ);
+    } else if (slab == 1) {
+      tz = tid % TZ, tx = (tid / TZ) % TX, ty = tid / (TX * TZ);
+    } else {
+      ty = tid % TY, tx = (tid / TY) % TX, tz = tid / (TX * TY);
+    }
     const int x = bx * TX + tx, y = by * TY + ty, z = bz * TZ + tz;
     const bool inside = x < C.ax[0].n && y < C.ax[1].n && z < C.ax[2].n;
     const int xc = x < C.ax[0].n ? x : C.ax[0].n - 1;
@@ -139,8 +154,8 @@ BatchLayout batch_layout(const bssn_grid *grids, int n) {
   BatchLayout L;
   L.desc = 0;
   L.prefix = L.desc + dwords((int64_t)4 * n * sizeof(BatchDesc));
-  // lean-tile prefix sums (n + 1), then face-tile prefix sums (n + 1)
-  L.proj = L.prefix + dwords((int64_t)2 * (n + 1) * sizeof(int));
+  // prefix sums (n + 1 each) of the lean list, then of the four face lists
+  L.proj = L.prefix + dwords((int64_t)5 * (n + 1) * sizeof(int));
   L.bufs = L.proj + dwords((int64_t)n * sizeof(ProjDesc));
   int64_t cells = 0;
   for (int i = 0; i < n; ++i) cells += (int64_t)grids[i].nx * grids[i].ny * grids[i].nz;
@@ -148,21 +163,36 @@ BatchLayout batch_layout(const bssn_grid *grids, int n) {
   return L;
 }
 
-template <typename T, bool CL>
-int launch_batch_t(int variant, const BatchDesc *D, const int *prefix, int ninst, int total, int blocks,
-                   hipStream_t st) {
+template <typename T, bool CL, int CLA>
+int launch_batch_t(int variant, const BatchDesc *D, const int *prefix, int ninst, int blocks, hipStream_t st) {
   switch (variant) {
     case 0:
-      hipLaunchKernelGGL((k_batch_tile<T, 4, 8, 8, CL>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      hipLaunchKernelGGL((k_batch_tile<T, 4, 8, 8, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst);
       break;
     case 1:
-      hipLaunchKernelGGL((k_batch_tile<T, 8, 8, 4, CL>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 8, 4, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst);
       break;
     default:
-      hipLaunchKernelGGL((k_batch_tile<T, 8, 4, 8, CL>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, total);
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 4, 8, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst);
       break;
   }
   return check_hip(hipGetLastError(), "batched stage kernel launch");
+}
+
+// one stage: the lean list, then the four face lists (disjoint points)
+template <typename T>
+int launch_batch_stage(int variant, const BatchDesc *D, const int *prefix, int ninst, int lblocks, int fblocks,
+                       hipStream_t st) {
+  int rc = BSSN_OK;
+  const int *fp = prefix + (ninst + 1);
+  if (lblocks > 0) rc = launch_batch_t<T, false, 7>(variant, D, prefix, ninst, lblocks, st);
+  if (fblocks > 0) {
+    if (!rc) rc = launch_batch_t<T, true, 1>(variant, D, fp, ninst, fblocks, st);
+    if (!rc) rc = launch_batch_t<T, true, 2>(variant, D, fp + (ninst + 1), ninst, fblocks, st);
+    if (!rc) rc = launch_batch_t<T, true, 4>(variant, D, fp + 2 * (ninst + 1), ninst, fblocks, st);
+    if (!rc) rc = launch_batch_t<T, true, 7>(variant, D, fp + 3 * (ninst + 1), ninst, fblocks, st);
+  }
+  return rc;
 }
 
 }  // namespace
@@ -188,9 +218,9 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
   if (grid_blocks < 1) return fail(BSSN_ERR_INVALID, "grid_blocks must be positive");
   const BatchLayout L = batch_layout(grids, n);
   std::vector<BatchDesc> descs((size_t)4 * n);
-  std::vector<int> prefix((size_t)2 * (n + 1));  // lean, then face
+  std::vector<int> prefix((size_t)5 * (n + 1));  // lean, then the four face lists
   std::vector<ProjDesc> proj((size_t)n);
-  int64_t tiles = 0, ftiles = 0, cells = 0, buf_off = L.bufs;
+  int64_t tiles = 0, ftiles[4] = {0, 0, 0, 0}, cells = 0, buf_off = L.bufs;
   const int t[3] = {variant_tile(tile_variant, 0), variant_tile(tile_variant, 1), variant_tile(tile_variant, 2)};
   for (int i = 0; i < n; ++i) {
     Ctx C;
@@ -211,12 +241,17 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
                   "instance %d: Sommerfeld faces in a batch must lie on the %d x %d x %d tile grid (whole grids)", i,
                   t[0], t[1], t[2]);
     const TileSet lean = tile_set(C, C.any_shell ? 1 : 0, t[0], t[1], t[2]);
-    TileSet face = tile_set(C, 2, t[0], t[1], t[2]);
-    if (!C.any_shell) face.count = 0;
-    if (tiles + lean.count >= ((int64_t)1 << 31) || ftiles + face.count >= ((int64_t)1 << 31))
-      return fail(BSSN_ERR_INVALID, "batch has too many tiles");
+    TileSet face[4];
+    for (int k = 0; k < 4; ++k) {
+      face[k] = tile_set(C, 3 + k, t[0], t[1], t[2]);
+      if (!C.any_shell) face[k].count = 0;
+    }
+    if (tiles + lean.count >= ((int64_t)1 << 31)) return fail(BSSN_ERR_INVALID, "batch has too many tiles");
     prefix[i] = (int)tiles;
-    prefix[(size_t)n + 1 + i] = (int)ftiles;
+    for (int k = 0; k < 4; ++k) {
+      if (ftiles[k] + face[k].count >= ((int64_t)1 << 31)) return fail(BSSN_ERR_INVALID, "batch has too many tiles");
+      prefix[(size_t)(k + 1) * (n + 1) + i] = (int)ftiles[k];
+    }
     double *u = states[i];
     double *ustar = work + buf_off, *ua = ustar + NCOMP * C.ncell, *ub = ua + NCOMP * C.ncell;
     buf_off += 3 * NCOMP * C.ncell;
@@ -230,17 +265,21 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
       stage_update(d.O, C.dt);
       d.us = ins[s];
       d.lean = lean;
-      d.face = face;
       d.lean_begin = (int)tiles;
-      d.face_begin = (int)ftiles;
+      for (int k = 0; k < 4; ++k) {
+        d.face[k] = face[k];
+        d.face_begin[k] = (int)ftiles[k];
+      }
     }
     proj[i] = ProjDesc{u, C.ncell, cells};
     tiles += lean.count;
-    ftiles += face.count;
+    for (int k = 0; k < 4; ++k) ftiles[k] += face[k].count;
     cells += C.ncell;
   }
   prefix[n] = (int)tiles;
-  prefix[(size_t)2 * n + 1] = (int)ftiles;
+  for (int k = 0; k < 4; ++k) prefix[(size_t)(k + 1) * (n + 1) + n] = (int)ftiles[k];
+  const int64_t fsum = ftiles[0] + ftiles[1] + ftiles[2] + ftiles[3];
+  if (fsum >= ((int64_t)1 << 31)) return fail(BSSN_ERR_INVALID, "batch has too many tiles");
   hipStream_t st = (hipStream_t)stream;
   char *base = (char *)work;
   int rc = check_hip(hipMemcpyAsync(base + L.desc * 8, descs.data(), descs.size() * sizeof(BatchDesc),
@@ -254,7 +293,7 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
   if (rc) return rc;
   plan->ninst = n;
   plan->total_tiles = (int32_t)tiles;
-  plan->face_tiles = (int32_t)ftiles;
+  plan->face_tiles = (int32_t)fsum;
   plan->precision = precision;
   plan->tile_variant = tile_variant;
   plan->grid_blocks = grid_blocks;
@@ -278,20 +317,13 @@ int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream) {
   const int64_t pblocks = (plan->total_cells + 255) / 256;
   hipLaunchKernelGGL(k_batch_project, dim3((unsigned)pblocks), dim3(256), 0, st, PD, plan->ninst, plan->total_cells);
   int rc = check_hip(hipGetLastError(), "batched projection launch");
-  const int *fprefix = prefix + plan->ninst + 1;
   const int lblocks = plan->grid_blocks < plan->total_tiles ? plan->grid_blocks : plan->total_tiles;
   const int fblocks = plan->grid_blocks < plan->face_tiles ? plan->grid_blocks : plan->face_tiles;
-  const bool f64 = plan->precision == BSSN_PREC_F64;
   for (int s = 0; s < 4 && !rc; ++s) {
-    // lean and face tiles write disjoint points of the stage's outputs
     const BatchDesc *Ds = D + (size_t)s * plan->ninst;
-    const int v = plan->tile_variant, ni = plan->ninst;
-    if (lblocks > 0)
-      rc = f64 ? launch_batch_t<double, false>(v, Ds, prefix, ni, plan->total_tiles, lblocks, st)
-               : launch_batch_t<float, false>(v, Ds, prefix, ni, plan->total_tiles, lblocks, st);
-    if (!rc && fblocks > 0)
-      rc = f64 ? launch_batch_t<double, true>(v, Ds, fprefix, ni, plan->face_tiles, fblocks, st)
-               : launch_batch_t<float, true>(v, Ds, fprefix, ni, plan->face_tiles, fblocks, st);
+    rc = plan->precision == BSSN_PREC_F64
+             ? launch_batch_stage<double>(plan->tile_variant, Ds, prefix, plan->ninst, lblocks, fblocks, st)
+             : launch_batch_stage<float>(plan->tile_variant, Ds, prefix, plan->ninst, lblocks, fblocks, st);
   }
   return rc;
 }
