@@ -174,3 +174,41 @@ def test_f32rhs_batch_matches_single_grid_f32rhs(J):
         print("f32rhs batch vs single f32rhs", f"{d32:.1e}", "vs f64", f"{d64:.1e}", flush=True)
         assert d32 <= 1e-12
         assert d64 <= 1e-5
+
+
+def test_sommerfeld_batch_graph_and_f32rhs(J):
+    """A batch with Sommerfeld instances (five launches per stage): graph
+    replays, with eager work in between, equal the eager batched steps; and
+    its FP32-RHS steps agree with each instance's single-grid FP32-RHS steps
+    (the same tile shape, so the same FP32 offsets; the face tiles of the two
+    paths are separate instantiations of the same code, which in FP64 agree
+    to 6e-15 and in FP32 to FP32 rounding: 1e-6) and stay within the
+    FP32-RHS error of FP64."""
+    from jax_bssn_amd.batch import BatchEvolver
+    gs, gp, _ = gauge_batch(J, (16,), seed=9)
+    ss, sp, _ = sommerfeld_instances(J, ((24, 24, 24), (16, 16, 32)), seed=60)
+    states, params = gs + ss, gp + sp
+    print("[batch sommerfeld graph / f32rhs]", flush=True)
+    eager = BatchEvolver([s.clone() for s in states], params, tile_variant=0, grid_blocks=256)
+    graphed = BatchEvolver([s.clone() for s in states], params, tile_variant=0, grid_blocks=256)
+    graphed.capture()
+    eager.step(1)
+    for rep in range(3):
+        graphed.step(1)
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device="cuda").sum().item()
+        eager.step(1)
+        torch.cuda.synchronize()
+        for a, e in zip(graphed.states, eager.states):
+            assert torch.equal(a, e), rep
+    b = BatchEvolver([s.clone() for s in states], params, precision="f32rhs", tile_variant=0, grid_blocks=256)
+    b.step(2)
+    for s, p, bs in zip(states, params, b.states):
+        e32 = J.Evolver(s.clone(), p, precision="f32rhs").step(2)
+        e64 = J.Evolver(s.clone(), p).step(2)
+        scale = float(e64.state.abs().max())
+        d32 = float((bs - e32.state).abs().max()) / scale
+        d64 = float((bs - e64.state).abs().max()) / scale
+        print("sommerfeld f32rhs batch vs single f32rhs", f"{d32:.1e}", "vs f64", f"{d64:.1e}", flush=True)
+        assert d32 <= 1e-6
+        assert d64 <= 1e-5
