@@ -3,14 +3,15 @@
 //
 // A prepared batch lives in a caller-owned work buffer: per instance its RK4
 // buffers and four stage descriptors (resolved parameters, stage buffers and
-// weights, tile counts, first tile), plus prefix sums of tiles and points.
-// Each stage is a launch of a fixed grid of workgroups that walk the global
-// list of lean tiles (tile t of instance i at prefix[i] + local index) and
-// run the same per-point RHS and in-register stage update as the
-// single-grid tile kernel, and, when some instance has Sommerfeld faces, a
-// second such launch over the global list of face tiles with the
-// closure-capable provider (one-sided rows and the radial face RHS).  A
-// step is a batched projection plus the stage launches, with no host work
+// weights, tile sets, first tile in each list), plus prefix sums of tiles
+// and points.  Each stage is a launch of a fixed grid of workgroups that walk
+// the global list of lean tiles (tile t of instance i at prefix[i] + local
+// index) and run the same per-point RHS and in-register stage update as the
+// single-grid tile kernel, and, when some instance has Sommerfeld faces, four
+// such launches over the global lists of its face tiles (slabs with closures
+// along x, y or z only, edges and corners) with the closure-capable
+// provider of that closure mask (one-sided rows and the radial face RHS).
+// A step is a batched projection plus the stage launches, with no host work
 // in between, so it can be captured in a HIP graph.
 //
 // Restricted to periodic or Sommerfeld faces on the tile grid of the
