@@ -87,7 +87,12 @@ __global__ void __launch_bounds__(TX *TY *TZ, (!CL && sizeof(T) == sizeof(double
   const int total = prefix[ninst];
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     const BatchDesc &d = D[batch_instance(prefix, ninst, t)];
-    const Ctx &C = d.C;
+    // the instance's context and outputs by value: kept in registers across
+    // the tile's barriers (whose fences would make every use of a reference
+    // into the descriptor table a reload)
+    const Ctx C = d.C;
+    const StageOut O = d.O;
+    const double *const us = d.us;
     int bx, by, bz, slab = 0;
     if (CL) tile_of(d.face[face_list(CLA)], t - d.face_begin[face_list(CLA)], bx, by, bz, &slab);
     else tile_of(d.lean, t - d.lean_begin, bx, by, bz);
@@ -106,14 +111,14 @@ __global__ void __launch_bounds__(TX *TY *TZ, (!CL && sizeof(T) == sizeof(double
     const int zc = z < C.ax[2].n ? z : C.ax[2].n - 1;
     const int64_t idx = (int64_t)xc * C.ax[0].stride + (int64_t)yc * C.ax[1].stride + zc;
     P pv;
-    pv.setup(C, d.us, smem, stash, bx * TX, by * TY, bz * TZ, tx, ty, tz);
+    pv.setup(C, us, smem, stash, bx * TX, by * TY, bz * TZ, tx, ty, tz);
     double u[NCOMP];
     {
-      const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)d.us;
+      const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)us;
 #pragma unroll
       for (int c = 0; c < NCOMP; ++c) u[c] = g[c * C.ncell + idx];
     }
-    Emitter<LdsStash<TX * TY * TZ>> out(C, d.O, idx, inside, pv.face(), pv.st);
+    Emitter<LdsStash<TX * TY * TZ>> out(C, O, idx, inside, pv.face(), pv.st);
     bssn_rhs_point<T>(pv, C, u, out);
     __syncthreads();  // the next tile restages the LDS box
   }
