@@ -239,7 +239,7 @@ def batched_punctures(J, steps, count=16, seed=1):
     """16 single punctures (Sommerfeld faces, 1+log, Gamma driver, the
     workload's parameters) of edge n in {32, 48, 64, 80, 96} on [-8, 8]^3
     with masses M ~ U(0.5, 1.5): one batched, autotuned, graph-captured RK4
-    step per replay (lean and face tiles in their own persistent launches).
+    step per replay (lean and face tiles in their own launches).
     Returns aggregate cell-updates/s and the largest per-instance difference
     to the same instance evolved alone."""
     import numpy as np

@@ -250,9 +250,9 @@ int bssn_tiled_ghost_fill(const bssn_tile_layout *layout, const int32_t *bc, int
 
 /* Batched evolution (no reference counterpart: the reference evolves one
  * grid per call; this advances many independent evolutions of uneven sizes
- * with persistent launches per RK4 stage: one over the lean tiles of every
- * instance, one over the face tiles of the instances with Sommerfeld
- * faces).  Each instance has its own parameters (periodic or Sommerfeld
+ * with shared launches per RK4 stage: one over the lean tiles of every
+ * instance, four over the face tiles of the instances with Sommerfeld
+ * faces, one workgroup per tile).  Each instance has its own parameters (periodic or Sommerfeld
  * faces, the Sommerfeld ones on the tile grid of the chosen tile shape,
  * whole grids, mad_q = 1, kappa = 0; others return BSSN_ERR_UNSUPPORTED)
  * and packed state.
@@ -260,15 +260,19 @@ int bssn_tiled_ghost_fill(const bssn_tile_layout *layout, const int32_t *bc, int
  * bssn_batch_prepare uploads the instance descriptors into work (doubles:
  * bssn_batch_work_size) and fills *plan; it waits for the upload.
  * tile_variant selects the tile shape (0: 4x8x8, 1: 8x8x4, 2: 8x4x8),
- * grid_blocks the number of persistent workgroups per stage launch.
+ * grid_blocks the most workgroups per launch (a list of more tiles is
+ * covered by successive launches).
  * bssn_batch_step then applies rk4_step (reference time_evolve.py:144-217)
  * once to every instance, in place: one projection launch and four stages
- * of one or two launches, nothing on the host in between (capturable in a
- * HIP graph).  plan->total_tiles / face_tiles count the lean / face tiles.
+ * of five tile lists each, nothing on the host in between (capturable in a
+ * HIP graph).  plan->total_tiles / face_tiles count the lean / face tiles,
+ * face_list_tiles the face tiles with closures along x, y, z only and the
+ * edge and corner tiles.
  * The states and work must stay where they were when the plan was made. */
 typedef struct bssn_batch_plan {
     int32_t ninst, total_tiles, precision, tile_variant, grid_blocks, face_tiles;
     int64_t total_cells, desc_offset, prefix_offset, proj_offset;
+    int32_t face_list_tiles[4];
 } bssn_batch_plan;
 
 int64_t bssn_batch_work_size(const bssn_grid *grids, int n);

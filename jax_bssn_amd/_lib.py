@@ -52,7 +52,7 @@ class CBatchPlan(ctypes.Structure):
     _fields_ = [("ninst", ctypes.c_int32), ("total_tiles", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("tile_variant", ctypes.c_int32), ("grid_blocks", ctypes.c_int32), ("face_tiles", ctypes.c_int32),
                 ("total_cells", ctypes.c_int64), ("desc_offset", ctypes.c_int64), ("prefix_offset", ctypes.c_int64),
-                ("proj_offset", ctypes.c_int64)]
+                ("proj_offset", ctypes.c_int64), ("face_list_tiles", ctypes.c_int32 * 4)]
 
 
 _SIGNATURES = {

@@ -1,6 +1,6 @@
-"""Many independent evolutions of uneven grid sizes in one persistent launch
-per RK4 stage, with HIP-graph-captured steps and an autotuner that picks
-the tile shape and the persistent grid size.
+"""Many independent evolutions of uneven grid sizes in shared launches per
+RK4 stage (one workgroup per tile of any instance), with HIP-graph-captured
+steps and an autotuner that picks the tile shape and the launch size.
 
 The reference evolves one grid per call (evolution/time_evolve.py:144-217);
 ``BatchEvolver.step`` applies the same ``rk4_step`` to every instance of the
@@ -61,7 +61,7 @@ class BatchEvolver:
                 tile_variant, grid_blocks = self.autotune()
                 _TUNED[key] = (tile_variant, grid_blocks)
             else:
-                tile_variant, grid_blocks = 0, 2 * _cu_count()
+                tile_variant, grid_blocks = 0, 1 << 20
                 self.tuning = {"source": "default"}
         self.prepare(tile_variant, grid_blocks)
         self.tuning.update(tile_variant=self.tile_variant, tile_shape=TILE_VARIANTS[self.tile_variant],
@@ -109,15 +109,15 @@ class BatchEvolver:
         return self
 
     def autotune(self, reps=2):
-        """Time every tile shape with a few persistent grid sizes on scratch
-        copies of the states; return the fastest (tile_variant, grid_blocks).
-        Tile shapes whose grid some instance's Sommerfeld faces are not on
-        are skipped."""
+        """Time every tile shape on scratch copies of the states, with each
+        tile list in one launch and in launches of 16 workgroups per CU;
+        return the fastest (tile_variant, grid_blocks).  Tile shapes whose
+        grid some instance's Sommerfeld faces are not on are skipped."""
         saved = [s.clone() for s in self.states]
         cus = _cu_count()
         best, best_t = None, float("inf")
         for v in TILE_VARIANTS:
-            for blocks in (cus, 2 * cus, 4 * cus, 1 << 20):
+            for blocks in (16 * cus, 1 << 20):
                 try:
                     self.prepare(v, blocks)
                 except NotImplementedError:

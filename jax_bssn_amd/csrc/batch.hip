@@ -1,10 +1,10 @@
 // Batched evolution: many independent evolutions of uneven grid sizes
-// advanced by persistent launches per RK4 stage.
+// advanced by shared launches per RK4 stage.
 //
 // A prepared batch lives in a caller-owned work buffer: per instance its RK4
 // buffers and four stage descriptors (resolved parameters, stage buffers and
 // weights, tile sets, first tile in each list), plus prefix sums of tiles
-// and points.  Each stage is a launch of a fixed grid of workgroups that walk
+// and points.  Each stage is a launch of one workgroup per tile of
 // the global list of lean tiles (tile t of instance i at prefix[i] + local
 // index) and run the same per-point RHS and in-register stage update as the
 // single-grid tile kernel, and, when some instance has Sommerfeld faces, four
@@ -75,23 +75,28 @@ __host__ __device__ constexpr int face_list(int cla) { return cla == 1 ? 0 : (cl
 // The lean tiles, compiled for the occupancy of the single-grid interior
 // tile kernel (one wave per SIMD in FP64, two in FP32); the face tiles (CL)
 // with the closure-capable provider and the wave orientation of the
-// single-grid face tiles.  Each walks its global list, whose length is the
-// last entry of its prefix sums.
+// single-grid face tiles.  One workgroup per tile: tile first + blockIdx.x
+// of a global list whose length is the last entry of its prefix sums.  (A
+// loop over tiles in one workgroup keeps every thread-derived LDS offset
+// live across iterations: 440 - 600 bytes of spills per lane, none without.)
 template <typename T, int TX, int TY, int TZ, bool CL, int CLA>
 __global__ void __launch_bounds__(TX *TY *TZ, (!CL && sizeof(T) == sizeof(double) ? 1 : 2))
-    k_batch_tile(const BatchDesc *__restrict__ D, const int *__restrict__ prefix, int ninst) {
+    k_batch_tile(const BatchDesc *__restrict__ D, const int *__restrict__ prefix, int ninst, int first) {
   using P = TileProvider<T, TX, TY, TZ, false, CL, CLA, !CL && sizeof(T) == sizeof(double)>;
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   const int tid = threadIdx.x;
-  const int total = prefix[ninst];
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
-    const BatchDesc &d = D[batch_instance(prefix, ninst, t)];
-    // the instance's context and outputs by value: kept in registers across
-    // the tile's barriers (whose fences would make every use of a reference
-    // into the descriptor table a reload)
-    const Ctx C = d.C;
-    const StageOut O = d.O;
+  const int t = first + (int)blockIdx.x;
+  if (t >= prefix[ninst]) return;  // block-uniform, before any barrier
+  // The descriptor table does not change while the launch runs: read it
+  // through the constant address space, as a kernel argument is read, so
+  // that the compiler may load a field again after the tile's barriers
+  // instead of holding the whole context in registers across them.
+  const BSSN_CONSTANT BatchDesc *Dk = (const BSSN_CONSTANT BatchDesc *)D;
+  {
+    const BatchDesc &d = *(const BatchDesc *)&Dk[batch_instance(prefix, ninst, t)];
+    const Ctx &C = d.C;
+    const StageOut &O = d.O;
     const double *const us = d.us;
     int bx, by, bz, slab = 0;
     if (CL) tile_of(d.face[face_list(CLA)], t - d.face_begin[face_list(CLA)], bx, by, bz, &slab);
@@ -120,7 +125,6 @@ __global__ void __launch_bounds__(TX *TY *TZ, (!CL && sizeof(T) == sizeof(double
     }
     Emitter<LdsStash<TX * TY * TZ>> out(C, O, idx, inside, pv.face(), pv.st);
     bssn_rhs_point<T>(pv, C, u, out);
-    __syncthreads();  // the next tile restages the LDS box
   }
 }
 
@@ -169,35 +173,46 @@ BatchLayout batch_layout(const bssn_grid *grids, int n) {
   return L;
 }
 
+// tiles [first, first + blocks) of one list
 template <typename T, bool CL, int CLA>
-int launch_batch_t(int variant, const BatchDesc *D, const int *prefix, int ninst, int blocks, hipStream_t st) {
+int launch_batch_t(int variant, const BatchDesc *D, const int *prefix, int ninst, int first, int blocks,
+                   hipStream_t st) {
   switch (variant) {
     case 0:
-      hipLaunchKernelGGL((k_batch_tile<T, 4, 8, 8, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst);
+      hipLaunchKernelGGL((k_batch_tile<T, 4, 8, 8, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, first);
       break;
     case 1:
-      hipLaunchKernelGGL((k_batch_tile<T, 8, 8, 4, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst);
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 8, 4, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, first);
       break;
     default:
-      hipLaunchKernelGGL((k_batch_tile<T, 8, 4, 8, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst);
+      hipLaunchKernelGGL((k_batch_tile<T, 8, 4, 8, CL, CLA>), dim3(blocks), dim3(256), 0, st, D, prefix, ninst, first);
       break;
   }
   return check_hip(hipGetLastError(), "batched stage kernel launch");
 }
 
+// every tile of one list, in launches of at most max_blocks workgroups
+template <typename T, bool CL, int CLA>
+int launch_batch_list(int variant, const BatchDesc *D, const int *prefix, int ninst, int tiles, int max_blocks,
+                      hipStream_t st) {
+  int rc = BSSN_OK;
+  for (int first = 0; first < tiles && !rc; first += max_blocks)
+    rc = launch_batch_t<T, CL, CLA>(variant, D, prefix, ninst, first,
+                                    tiles - first < max_blocks ? tiles - first : max_blocks, st);
+  return rc;
+}
+
 // one stage: the lean list, then the four face lists (disjoint points)
 template <typename T>
-int launch_batch_stage(int variant, const BatchDesc *D, const int *prefix, int ninst, int lblocks, int fblocks,
-                       hipStream_t st) {
-  int rc = BSSN_OK;
-  const int *fp = prefix + (ninst + 1);
-  if (lblocks > 0) rc = launch_batch_t<T, false, 7>(variant, D, prefix, ninst, lblocks, st);
-  if (fblocks > 0) {
-    if (!rc) rc = launch_batch_t<T, true, 1>(variant, D, fp, ninst, fblocks, st);
-    if (!rc) rc = launch_batch_t<T, true, 2>(variant, D, fp + (ninst + 1), ninst, fblocks, st);
-    if (!rc) rc = launch_batch_t<T, true, 4>(variant, D, fp + 2 * (ninst + 1), ninst, fblocks, st);
-    if (!rc) rc = launch_batch_t<T, true, 7>(variant, D, fp + 3 * (ninst + 1), ninst, fblocks, st);
-  }
+int launch_batch_stage(const bssn_batch_plan *plan, const BatchDesc *D, const int *prefix, hipStream_t st) {
+  const int v = plan->tile_variant, n = plan->ninst, mb = plan->grid_blocks;
+  const int *fp = prefix + (n + 1);
+  const int *ft = plan->face_list_tiles;
+  int rc = launch_batch_list<T, false, 7>(v, D, prefix, n, plan->total_tiles, mb, st);
+  if (!rc) rc = launch_batch_list<T, true, 1>(v, D, fp, n, ft[face_list(1)], mb, st);
+  if (!rc) rc = launch_batch_list<T, true, 2>(v, D, fp + (n + 1), n, ft[face_list(2)], mb, st);
+  if (!rc) rc = launch_batch_list<T, true, 4>(v, D, fp + 2 * (n + 1), n, ft[face_list(4)], mb, st);
+  if (!rc) rc = launch_batch_list<T, true, 7>(v, D, fp + 3 * (n + 1), n, ft[face_list(7)], mb, st);
   return rc;
 }
 
@@ -300,6 +315,7 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
   plan->ninst = n;
   plan->total_tiles = (int32_t)tiles;
   plan->face_tiles = (int32_t)fsum;
+  for (int k = 0; k < 4; ++k) plan->face_list_tiles[k] = (int32_t)ftiles[k];
   plan->precision = precision;
   plan->tile_variant = tile_variant;
   plan->grid_blocks = grid_blocks;
@@ -312,9 +328,10 @@ int bssn_batch_prepare(const bssn_params *params, const bssn_grid *grids, double
 
 int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream) {
   if (!plan || !work) return fail(BSSN_ERR_INVALID, "null argument");
-  if (plan->ninst < 1 || plan->total_tiles < 0 || plan->face_tiles < 0 ||
-      plan->total_tiles + plan->face_tiles < 1 || plan->grid_blocks < 1 || plan->tile_variant < 0 ||
-      plan->tile_variant >= BATCH_VARIANTS)
+  const int32_t *ft = plan->face_list_tiles;
+  if (plan->ninst < 1 || plan->total_tiles < 0 || ft[0] < 0 || ft[1] < 0 || ft[2] < 0 || ft[3] < 0 ||
+      (int64_t)ft[0] + ft[1] + ft[2] + ft[3] != plan->face_tiles || plan->total_tiles + plan->face_tiles < 1 ||
+      plan->grid_blocks < 1 || plan->tile_variant < 0 || plan->tile_variant >= BATCH_VARIANTS)
     return fail(BSSN_ERR_INVALID, "batch plan is not prepared");
   hipStream_t st = (hipStream_t)stream;
   const BatchDesc *D = (const BatchDesc *)(work + plan->desc_offset);
@@ -323,13 +340,10 @@ int bssn_batch_step(const bssn_batch_plan *plan, double *work, void *stream) {
   const int64_t pblocks = (plan->total_cells + 255) / 256;
   hipLaunchKernelGGL(k_batch_project, dim3((unsigned)pblocks), dim3(256), 0, st, PD, plan->ninst, plan->total_cells);
   int rc = check_hip(hipGetLastError(), "batched projection launch");
-  const int lblocks = plan->grid_blocks < plan->total_tiles ? plan->grid_blocks : plan->total_tiles;
-  const int fblocks = plan->grid_blocks < plan->face_tiles ? plan->grid_blocks : plan->face_tiles;
   for (int s = 0; s < 4 && !rc; ++s) {
     const BatchDesc *Ds = D + (size_t)s * plan->ninst;
-    rc = plan->precision == BSSN_PREC_F64
-             ? launch_batch_stage<double>(plan->tile_variant, Ds, prefix, plan->ninst, lblocks, fblocks, st)
-             : launch_batch_stage<float>(plan->tile_variant, Ds, prefix, plan->ninst, lblocks, fblocks, st);
+    rc = plan->precision == BSSN_PREC_F64 ? launch_batch_stage<double>(plan, Ds, prefix, st)
+                                          : launch_batch_stage<float>(plan, Ds, prefix, st);
   }
   return rc;
 }

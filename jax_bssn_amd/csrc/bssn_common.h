@@ -15,9 +15,11 @@
 #if defined(__HIP_DEVICE_COMPILE__)
 #define BSSN_LDS __attribute__((address_space(3)))
 #define BSSN_GLOBAL __attribute__((address_space(1)))
+#define BSSN_CONSTANT __attribute__((address_space(4)))
 #else
 #define BSSN_LDS
 #define BSSN_GLOBAL
+#define BSSN_CONSTANT
 #endif
 
 namespace bssn {

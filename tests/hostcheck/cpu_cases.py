@@ -18,8 +18,9 @@ meant to run, so that every kernel family takes part:
   and on the point kernels (z-Sommerfeld, kappa = 0.02);
 * tangent_tiles / tangent_points: one- and two-direction tangents against
   central differences of the CPU build's own RHS;
-* batch: two uneven periodic instances in one persistent launch against
-  single-grid steps;
+* batch: two uneven periodic instances and a Sommerfeld one in the batched
+  launches (lists split into launches of 64 workgroups) against single-grid
+  steps;
 * poisoned_guards: NaN written into every sample beyond the stencils'
   reach of a tiled layout's owned points leaves the owned RHS unchanged.
 """
@@ -290,8 +291,9 @@ def case_tangent_points(L):
 
 def case_batch(L):
     """Two uneven periodic gauge-wave-like instances and a Sommerfeld one
-    (lean and face tiles) advanced by the batched persistent launches against
-    the same steps one grid at a time."""
+    (lean and face tiles) advanced by the batched launches, each tile list
+    covered by launches of at most 64 workgroups, against the same steps one
+    grid at a time."""
     from jax_bssn_amd import _lib as LB
     shapes = ((16, 16, 16), (24, 16, 32), (24, 16, 32))
     states, params = [], []

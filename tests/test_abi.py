@@ -46,6 +46,9 @@ def test_struct_layout_matches_header():
     assert ctypes.sizeof(_lib.CParams) == 10 * 8 + 21 * 4 + 4  # padded to 8
     assert _lib.CParams.bc.offset == 88
     assert ctypes.sizeof(_lib.CGrid) == 12
+    # 6 int32, 4 int64, then the 4 per-list face tile counts
+    assert ctypes.sizeof(_lib.CBatchPlan) == 6 * 4 + 4 * 8 + 4 * 4
+    assert _lib.CBatchPlan.total_cells.offset == 24 and _lib.CBatchPlan.face_list_tiles.offset == 56
 
 
 def test_version_string(lib):

@@ -1,4 +1,4 @@
-"""GPU checks of the batched evolution (one persistent launch per stage).
+"""GPU checks of the batched evolution (shared launches per stage).
 
 * every instance of a batch of uneven periodic gauge waves equals its own
   single-grid evolution after three steps (the same per-point code: FP64
@@ -132,7 +132,7 @@ def sommerfeld_instances(J, shapes, seed=50):
 def test_batch_with_sommerfeld_instances(J, variant):
     """Periodic gauge waves and Sommerfeld instances (faces on every tile
     shape's grid) in one batch: lean tiles and face tiles in their own
-    persistent launches; each instance equals its single-grid evolution
+    launches; each instance equals its single-grid evolution
     (1e-14) and a Sommerfeld one matches the oracle step by step (1e-12)."""
     from jax_bssn_amd.batch import BatchEvolver
     gs, gp, _ = gauge_batch(J, (16, 24), seed=3)
