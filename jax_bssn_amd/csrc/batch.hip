@@ -75,8 +75,9 @@ __host__ __device__ constexpr int face_list(int cla) { return cla == 1 ? 0 : (cl
 // The lean tiles, compiled for the occupancy of the single-grid interior
 // tile kernel (one wave per SIMD in FP64, two in FP32); the face tiles (CL)
 // with the closure-capable provider and the wave orientation of the
-// single-grid face tiles.  One workgroup per tile: tile first + blockIdx.x
-// of a global list whose length is the last entry of its prefix sums.  (A
+// single-grid face tiles.  One workgroup per tile: tile first + (blockIdx.x
+// remapped per XCD) of a global list whose length is the last entry of its
+// prefix sums.  (A
 // loop over tiles in one workgroup keeps every thread-derived LDS offset
 // live across iterations: 440 - 600 bytes of spills per lane, none without.)
 template <typename T, int TX, int TY, int TZ, bool CL, int CLA>
@@ -86,7 +87,9 @@ __global__ void __launch_bounds__(TX *TY *TZ, (!CL && sizeof(T) == sizeof(double
   __shared__ T smem[P::LDS_ELEMS];
   __shared__ double stash[STASH_SLOTS * TX * TY * TZ];
   const int tid = threadIdx.x;
-  const int t = first + (int)blockIdx.x;
+  // each XCD takes a contiguous range of the launch's tiles (neighbouring
+  // tiles of an instance share halos in one L2)
+  const int t = first + xcd_remap((int)blockIdx.x, (int)gridDim.x);
   if (t >= prefix[ninst]) return;  // block-uniform, before any barrier
   // The descriptor table does not change while the launch runs: read it
   // through the constant address space, as a kernel argument is read, so

@@ -35,15 +35,6 @@ constexpr int tile_min_waves() {
 
 namespace bssn {
 
-// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of tiles so tiles sharing halos share an L2.
-__device__ __forceinline__ int xcd_remap(int i, int n) {
-  constexpr int X = 8;
-  const int per = n / X, rem = n % X;
-  const int x = i % X, l = i / X;
-  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
-}
-
 __device__ __forceinline__ void load_point(const Ctx &C, const double *__restrict__ u, int64_t idx,
                                            double (&v)[NCOMP]) {
   const BSSN_GLOBAL double *g = (const BSSN_GLOBAL double *)u;

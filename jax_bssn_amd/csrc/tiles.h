@@ -9,6 +9,16 @@
 
 namespace bssn {
 
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so tiles sharing halos share an L2 (a
+// permutation of [0, n) for 0 <= i < n).
+BSSN_HD __forceinline__ int xcd_remap(int i, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X;
+  const int x = i % X, l = i / X;
+  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+}
+
 // A tile whose box meets a closure point (the three planes next to a
 // Sommerfeld face) needs the closure-capable kernel.  Along each axis those
 // tiles form at most two index ranges, [l0, l1) at the low face and

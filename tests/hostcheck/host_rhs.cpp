@@ -71,6 +71,22 @@ extern "C" int host_stage(const bssn_params *p, const bssn_grid *g, int stage, c
 // count of the two sets adds up to the grid, and kinds 3..6 enumerate the
 // closure tiles once each, split by the axes of their closures.  Returns the number of
 // violations (or a negative status).
+// The XCD remapping of workgroup indexes: count the n in [1, nmax] for
+// which it is not a permutation of [0, n).
+extern "C" int host_xcd_remap_check(int nmax) {
+  int bad = 0;
+  for (int n = 1; n <= nmax; ++n) {
+    std::vector<int> seen((size_t)n, 0);
+    bool ok = true;
+    for (int i = 0; i < n; ++i) {
+      const int t = xcd_remap(i, n);
+      if (t < 0 || t >= n || seen[(size_t)t]++) ok = false;
+    }
+    bad += ok ? 0 : 1;
+  }
+  return bad;
+}
+
 extern "C" int host_tile_check(const bssn_params *p, const bssn_grid *g, int tx, int ty, int tz, char *err,
                                int errlen) {
   Ctx C;

@@ -148,6 +148,13 @@ def test_host_rhs_f32_within_stated_error(host, case):
     assert max(errs.values()) <= F32_RHS_TOL, errs
 
 
+def test_xcd_remap_is_a_permutation(host):
+    """Every launch of n workgroups (n up to 5000, and the 8 XCDs' share
+    uneven) visits each of its n tiles exactly once after the per-XCD
+    remapping."""
+    assert host.host_xcd_remap_check(5000) == 0
+
+
 @pytest.mark.parametrize("tile", [(4, 8, 8), (8, 8, 8), (2, 8, 16), (8, 8, 4)])
 def test_tile_sets_partition_the_grid(host, tile):
     """The lean / closure tile launches cover every tile exactly once and
